@@ -1,0 +1,478 @@
+// Plan construction and the C ABI (include/hmat_amd.h).
+//
+// hm_plan_create turns the exported block table into device work lists:
+//   * the arena is converted once to the storage precision and uploaded;
+//   * dense leaves get a second, transposed copy so that H' x runs through
+//     the same row-owner kernel as H x (the transposed product is then as
+//     bandwidth-friendly as the forward one; low-rank factors are shared,
+//     with A and B swapping roles);
+//   * rows are cut at every leaf boundary and then into tiles of <= 64 rows;
+//     every tile lists the leaves that cross it, in block-table order;
+//   * low-rank leaves get projection items of <= PROJ_CHUNK rows and
+//     <= PROJ_KMAX rank columns.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+
+#include "../../include/hmat_amd.h"
+#include "hm_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+struct HipFail {
+    hipError_t e;
+    std::string where;
+};
+
+void hip_check(hipError_t e, const char *where) {
+    if (e != hipSuccess) throw HipFail{e, where};
+}
+
+struct OB {  // oriented leaf
+    int kind;
+    int64_t row0, col0, m, n, k, offA, offB;
+};
+
+uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) {  // inf / nan: keep class
+        uint16_t h = static_cast<uint16_t>(u >> 16);
+        if (u & 0x007fffffu) h |= 0x0040;
+        return h;
+    }
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return static_cast<uint16_t>(u >> 16);
+}
+
+float bf16_to_f32_host(uint16_t h) {
+    uint32_t u = static_cast<uint32_t>(h) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+template <class T>
+T *dev_upload(hm::Plan &p, const std::vector<T> &v) {
+    if (v.empty()) return nullptr;
+    void *d = nullptr;
+    const size_t bytes = v.size() * sizeof(T);
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) throw HipFail{e, "hipMalloc"};
+    p.allocs.push_back(d);
+    p.device_bytes += static_cast<int64_t>(bytes);
+    hip_check(hipMemcpy(d, v.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    return static_cast<T *>(d);
+}
+
+void *dev_alloc(hm::Plan &p, size_t bytes) {
+    if (bytes == 0) return nullptr;
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) throw HipFail{e, "hipMalloc"};
+    p.allocs.push_back(d);
+    p.device_bytes += static_cast<int64_t>(bytes);
+    return d;
+}
+
+size_t storage_size(int st) { return st == HM_F64 ? 8 : st == HM_F32 ? 4 : 2; }
+size_t acc_size(int st) { return st == HM_F64 ? 8 : 4; }
+
+// Convert a double array to the storage arena(s) on the host.
+void pack_storage(int st, const double *src, int64_t n, std::vector<uint8_t> &hi,
+                  std::vector<uint8_t> &lo) {
+    hi.resize(static_cast<size_t>(n) * storage_size(st));
+    if (st == HM_F64) {
+        std::memcpy(hi.data(), src, static_cast<size_t>(n) * 8);
+    } else if (st == HM_F32) {
+        float *o = reinterpret_cast<float *>(hi.data());
+        for (int64_t i = 0; i < n; ++i) o[i] = static_cast<float>(src[i]);
+    } else {
+        uint16_t *o = reinterpret_cast<uint16_t *>(hi.data());
+        uint16_t *ol = nullptr;
+        if (st == HM_BF16X2) {
+            lo.resize(static_cast<size_t>(n) * 2);
+            ol = reinterpret_cast<uint16_t *>(lo.data());
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            const uint16_t h = f32_to_bf16_rne(static_cast<float>(src[i]));
+            o[i] = h;
+            if (ol) ol[i] = f32_to_bf16_rne(static_cast<float>(src[i] - bf16_to_f32_host(h)));
+        }
+    }
+}
+
+void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
+                     const std::vector<OB> &obs) {
+    using namespace hm;
+    d.nrows = nrows;
+    d.ncols = ncols;
+    const int64_t nb = static_cast<int64_t>(obs.size());
+    std::vector<BlkDev> blk(obs.size());
+    std::vector<ProjItem> proj;
+    std::vector<RedItem> red;
+    // t offsets
+    int64_t tLen = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+        const OB &o = obs[b];
+        BlkDev &x = blk[b];
+        x.offA = o.offA;
+        x.offB = o.offB;
+        x.row0 = o.row0;
+        x.col0 = o.col0;
+        x.m = static_cast<int32_t>(o.m);
+        x.n = static_cast<int32_t>(o.n);
+        x.k = static_cast<int32_t>(o.k);
+        x.kind = o.kind;
+        x.tOff = 0;
+        if (o.kind == KIND_RK && o.k > 0) {
+            x.tOff = tLen;
+            tLen += o.k;
+        }
+    }
+    // projection items; partial offsets are placed after t
+    int64_t partLen = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+        const OB &o = obs[b];
+        if (o.kind != KIND_RK || o.k == 0 || o.n == 0) continue;
+        const int64_t nchunk = (o.n + PROJ_CHUNK - 1) / PROJ_CHUNK;
+        for (int64_t j0 = 0; j0 < o.k; j0 += PROJ_KMAX) {
+            const int32_t kc = static_cast<int32_t>(std::min<int64_t>(PROJ_KMAX, o.k - j0));
+            const int64_t tDst = blk[b].tOff + j0;
+            int64_t pOff = 0;
+            if (nchunk > 1) {
+                pOff = partLen;
+                partLen += nchunk * kc;
+                red.push_back(RedItem{pOff, tDst, static_cast<int32_t>(nchunk), kc});
+            }
+            for (int64_t c = 0; c < nchunk; ++c) {
+                ProjItem it{};
+                it.blk = static_cast<int32_t>(b);
+                it.row0 = static_cast<int32_t>(c * PROJ_CHUNK);
+                it.rows = static_cast<int32_t>(std::min<int64_t>(PROJ_CHUNK, o.n - c * PROJ_CHUNK));
+                it.j0 = static_cast<int32_t>(j0);
+                it.kc = kc;
+                it.out = nchunk > 1 ? tLen + pOff + c * kc : tDst;
+                proj.push_back(it);
+            }
+        }
+    }
+    // row tiles: cut at every leaf row boundary, then into <= TILE_ROWS pieces
+    std::vector<int64_t> cuts;
+    cuts.reserve(2 * obs.size() + 2);
+    cuts.push_back(0);
+    cuts.push_back(nrows);
+    for (const OB &o : obs) {
+        cuts.push_back(o.row0);
+        cuts.push_back(o.row0 + o.m);
+    }
+    std::sort(cuts.begin(), cuts.end());
+    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+    std::vector<TileDev> tiles;
+    for (size_t i = 0; i + 1 < cuts.size(); ++i) {
+        for (int64_t r = cuts[i]; r < cuts[i + 1]; r += TILE_ROWS) {
+            TileDev t{};
+            t.row0 = r;
+            t.rows = static_cast<int32_t>(std::min<int64_t>(TILE_ROWS, cuts[i + 1] - r));
+            tiles.push_back(t);
+        }
+    }
+    // tile -> contributing leaves (CSR), leaves in table order
+    std::vector<int64_t> tileStart(tiles.size());
+    for (size_t i = 0; i < tiles.size(); ++i) tileStart[i] = tiles[i].row0;
+    std::vector<int32_t> cnt(tiles.size() + 1, 0);
+    auto first_tile = [&](int64_t row) {
+        return static_cast<size_t>(std::lower_bound(tileStart.begin(), tileStart.end(), row) -
+                                   tileStart.begin());
+    };
+    auto contributes = [](const OB &o) {
+        return o.m > 0 && o.n > 0 && (o.kind == KIND_DENSE || o.k > 0);
+    };
+    for (const OB &o : obs) {
+        if (!contributes(o)) continue;
+        for (size_t t = first_tile(o.row0); t < tiles.size() && tiles[t].row0 < o.row0 + o.m; ++t)
+            cnt[t + 1]++;
+    }
+    for (size_t t = 0; t < tiles.size(); ++t) cnt[t + 1] += cnt[t];
+    std::vector<TileItem> items(static_cast<size_t>(cnt[tiles.size()]));
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t b = 0; b < nb; ++b) {
+        const OB &o = obs[b];
+        if (!contributes(o)) continue;
+        for (size_t t = first_tile(o.row0); t < tiles.size() && tiles[t].row0 < o.row0 + o.m; ++t)
+            items[fill[t]++] = TileItem{static_cast<int32_t>(b),
+                                        static_cast<int32_t>(tiles[t].row0 - o.row0)};
+    }
+    for (size_t t = 0; t < tiles.size(); ++t) {
+        tiles[t].ib = cnt[t];
+        tiles[t].ie = cnt[t + 1];
+    }
+    d.blk = dev_upload(p, blk);
+    d.nblk = nb;
+    d.proj = dev_upload(p, proj);
+    d.nproj = static_cast<int64_t>(proj.size());
+    d.red = dev_upload(p, red);
+    d.nred = static_cast<int64_t>(red.size());
+    d.tiles = dev_upload(p, tiles);
+    d.ntiles = static_cast<int64_t>(tiles.size());
+    d.items = dev_upload(p, items);
+    d.nitems = static_cast<int64_t>(items.size());
+    d.tLen = tLen;
+    d.partLen = partLen;
+    d.t = dev_alloc(p, static_cast<size_t>(tLen + partLen) * acc_size(p.storage));
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+void free_plan(hm::Plan *p) {
+    if (!p) return;
+    DeviceGuard g(p->device);
+    for (void *a : p->allocs) (void)hipFree(a);
+    delete p;
+}
+
+int check_vectors(const hm::Plan *p, const void *x, const void *y) {
+    if (!p) return fail(HM_EINVAL, "plan is NULL");
+    if ((p->ncols > 0 && !x) || (p->nrows > 0 && !y)) return fail(HM_EINVAL, "NULL vector");
+    return HM_OK;
+}
+
+int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void *X, int64_t ldx,
+                  void *Y, int64_t ldy, double alpha, double beta, void *stream) {
+    if (nrhs < 0) return fail(HM_EINVAL, "nrhs < 0");
+    if (nrhs > 0 && (ldx < d.ncols || ldy < d.nrows || ldx < 1 || ldy < 1))
+        return fail(HM_EINVAL, "leading dimension smaller than the vector length");
+    if (!std::isfinite(alpha) || !std::isfinite(beta))
+        return fail(HM_EINVAL, "alpha and beta must be finite");
+    if (nrhs > 0 && ((d.ncols > 0 && !X) || (d.nrows > 0 && !Y)))
+        return fail(HM_EINVAL, "NULL vector");
+    if (nrhs > 0 && d.nrows > 0 && d.ncols > 0) {
+        const size_t es = acc_size(p->storage);
+        const char *xb = static_cast<const char *>(X);
+        const char *yb = static_cast<const char *>(Y);
+        const char *xe = xb + ((nrhs - 1) * ldx + d.ncols) * es;
+        const char *ye = yb + ((nrhs - 1) * ldy + d.nrows) * es;
+        if (xb < ye && yb < xe) return fail(HM_EINVAL, "x and y overlap");
+    }
+    DeviceGuard g(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t es = acc_size(p->storage);
+    for (int64_t c = 0; c < nrhs; ++c) {
+        const void *xc = static_cast<const char *>(X) + c * ldx * es;
+        void *yc = static_cast<char *>(Y) + c * ldy * es;
+        hipError_t e = hm::launch_matvec(*p, d, xc, yc, alpha, beta, s);
+        if (e != hipSuccess)
+            return fail(HM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    }
+    return HM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hm_last_error(void) { return g_err.c_str(); }
+
+const char *hm_version(void) { return "hmat_amd 0.1 (gfx950)"; }
+
+int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *blocks,
+                   const double *data, int64_t ndata, int storage, hm_plan_t *plan) {
+    using namespace hm;
+    g_err.clear();
+    if (!plan) return fail(HM_EINVAL, "plan output pointer is NULL");
+    *plan = nullptr;
+    if (nrows < 0 || ncols < 0 || nblocks < 0 || ndata < 0)
+        return fail(HM_EINVAL, "negative size");
+    if (nrows >= (int64_t(1) << 31) || ncols >= (int64_t(1) << 31))
+        return fail(HM_EINVAL, "matrix dimension must be < 2^31");
+    if (storage < HM_F64 || storage > HM_BF16X2) return fail(HM_EINVAL, "unknown storage");
+    if (nblocks > 0 && !blocks) return fail(HM_EINVAL, "blocks is NULL");
+    if (ndata > 0 && !data) return fail(HM_EINVAL, "data is NULL");
+    if (nblocks >= (int64_t(1) << 31)) return fail(HM_EINVAL, "too many blocks");
+
+    std::vector<OB> fw(static_cast<size_t>(nblocks));
+    std::vector<int64_t> rowCover(static_cast<size_t>(nrows) + 1, 0);
+    int64_t area = 0, ndense = 0, nrk = 0, dense_elems = 0, rk_elems = 0, maxrank = 0;
+    for (int64_t b = 0; b < nblocks; ++b) {
+        const int64_t *r = blocks + 8 * b;
+        OB o{static_cast<int>(r[0]), r[1], r[2], r[3], r[4], r[5], r[6], r[7]};
+        char msg[160];
+        if (o.kind != KIND_DENSE && o.kind != KIND_RK) {
+            std::snprintf(msg, sizeof msg, "block %lld: unknown kind %d", (long long)b, o.kind);
+            return fail(HM_EINVAL, msg);
+        }
+        if (o.row0 < 0 || o.col0 < 0 || o.m < 0 || o.n < 0 || o.k < 0 || o.offA < 0 || o.offB < 0 ||
+            o.row0 + o.m > nrows || o.col0 + o.n > ncols) {
+            std::snprintf(msg, sizeof msg, "block %lld: outside the %lldx%lld matrix", (long long)b,
+                          (long long)nrows, (long long)ncols);
+            return fail(HM_EINVAL, msg);
+        }
+        if (o.kind == KIND_DENSE) {
+            if (o.offA + o.m * o.n > ndata) {
+                std::snprintf(msg, sizeof msg, "block %lld: dense leaf outside the arena", (long long)b);
+                return fail(HM_EINVAL, msg);
+            }
+            ++ndense;
+            dense_elems += o.m * o.n;
+        } else {
+            if (o.k >= (int64_t(1) << 20) || o.offA + o.m * o.k > ndata || o.offB + o.n * o.k > ndata) {
+                std::snprintf(msg, sizeof msg, "block %lld: low-rank factors outside the arena",
+                              (long long)b);
+                return fail(HM_EINVAL, msg);
+            }
+            ++nrk;
+            rk_elems += (o.m + o.n) * o.k;
+            maxrank = std::max(maxrank, o.k);
+        }
+        area += o.m * o.n;
+        rowCover[o.row0] += o.n;
+        rowCover[o.row0 + o.m] -= o.n;
+        fw[b] = o;
+    }
+    if (area != nrows * ncols) return fail(HM_EINVAL, "leaves do not tile the matrix");
+    {
+        int64_t acc = 0;
+        for (int64_t i = 0; i < nrows; ++i) {
+            acc += rowCover[i];
+            if (acc != ncols) {
+                char msg[120];
+                std::snprintf(msg, sizeof msg, "row %lld is covered by %lld columns, expected %lld",
+                              (long long)i, (long long)acc, (long long)ncols);
+                return fail(HM_EINVAL, msg);
+            }
+        }
+    }
+
+    std::unique_ptr<Plan> p(new (std::nothrow) Plan());
+    if (!p) return fail(HM_ENOMEM, "host allocation failed");
+    if (hipGetDevice(&p->device) != hipSuccess) return fail(HM_EHIP, "no HIP device");
+    p->storage = storage;
+    p->nrows = nrows;
+    p->ncols = ncols;
+    p->ndense = ndense;
+    p->nrk = nrk;
+    p->dense_elems = dense_elems;
+    p->rk_elems = rk_elems;
+    p->maxrank = maxrank;
+    try {
+        // forward arena
+        std::vector<uint8_t> hi, lo;
+        pack_storage(storage, data, ndata, hi, lo);
+        p->fac = dev_upload(*p, hi);
+        p->fac_lo = lo.empty() ? nullptr : dev_upload(*p, lo);
+        // transposed dense leaves, packed compactly
+        std::vector<double> dT(static_cast<size_t>(dense_elems));
+        std::vector<OB> bw(fw.size());
+        int64_t off = 0;
+        for (size_t b = 0; b < fw.size(); ++b) {
+            const OB &o = fw[b];
+            OB t{o.kind, o.col0, o.row0, o.n, o.m, o.k, o.offB, o.offA};
+            if (o.kind == KIND_DENSE) {
+                const double *C = data + o.offA;
+                for (int64_t j = 0; j < o.n; ++j)
+                    for (int64_t i = 0; i < o.m; ++i) dT[off + j + i * o.n] = C[i + j * o.m];
+                t.offA = off;
+                t.offB = 0;
+                off += o.m * o.n;
+            }
+            bw[b] = t;
+        }
+        std::vector<uint8_t> thi, tlo;
+        pack_storage(storage, dT.data(), dense_elems, thi, tlo);
+        p->facT = dev_upload(*p, thi);
+        p->facT_lo = tlo.empty() ? nullptr : dev_upload(*p, tlo);
+        build_direction(*p, p->fwd, nrows, ncols, fw);
+        p->fwd.denseA = p->fac;
+        p->fwd.denseA_lo = p->fac_lo;
+        build_direction(*p, p->bwd, ncols, nrows, bw);
+        p->bwd.denseA = p->facT;
+        p->bwd.denseA_lo = p->facT_lo;
+    } catch (const HipFail &h) {
+        const int code = (h.e == hipErrorOutOfMemory || h.e == hipErrorMemoryAllocation) ? HM_ENOMEM : HM_EHIP;
+        std::string msg = h.where + ": " + hipGetErrorString(h.e);
+        free_plan(p.release());
+        return fail(code, msg);
+    } catch (const std::bad_alloc &) {
+        free_plan(p.release());
+        return fail(HM_ENOMEM, "host allocation failed");
+    }
+    *plan = reinterpret_cast<hm_plan_t>(p.release());
+    return HM_OK;
+}
+
+int hm_plan_destroy(hm_plan_t plan) {
+    free_plan(reinterpret_cast<hm::Plan *>(plan));
+    return HM_OK;
+}
+
+int hm_plan_info(hm_plan_t plan, int64_t *out, int nout) {
+    const hm::Plan *p = reinterpret_cast<const hm::Plan *>(plan);
+    if (!p || (!out && nout > 0)) return fail(HM_EINVAL, "NULL argument");
+    const int64_t es = static_cast<int64_t>(storage_size(p->storage)) * (p->storage == HM_BF16X2 ? 2 : 1);
+    const int64_t v[HM_INFO_NFIELDS] = {p->nrows,
+                                        p->ncols,
+                                        p->ndense,
+                                        p->nrk,
+                                        p->dense_elems,
+                                        p->rk_elems,
+                                        p->storage,
+                                        p->device_bytes,
+                                        (p->dense_elems + p->rk_elems) * es,
+                                        p->maxrank};
+    for (int i = 0; i < nout && i < HM_INFO_NFIELDS; ++i) out[i] = v[i];
+    return HM_OK;
+}
+
+int hm_matvec(hm_plan_t plan, const void *x, void *y, double alpha, double beta, void *stream) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    int rc = check_vectors(p, x, y);
+    if (rc) return rc;
+    return run_direction(p, p->fwd, 1, x, p->ncols > 0 ? p->ncols : 1, y,
+                         p->nrows > 0 ? p->nrows : 1, alpha, beta, stream);
+}
+
+int hm_matmat(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx, void *Y, int64_t ldy,
+              double alpha, double beta, void *stream) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    if (!p) return fail(HM_EINVAL, "plan is NULL");
+    return run_direction(p, p->fwd, nrhs, X, ldx, Y, ldy, alpha, beta, stream);
+}
+
+int hm_matvec_t(hm_plan_t plan, const void *x, void *y, double alpha, double beta, void *stream) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    if (!p) return fail(HM_EINVAL, "plan is NULL");
+    if ((p->nrows > 0 && !x) || (p->ncols > 0 && !y)) return fail(HM_EINVAL, "NULL vector");
+    return run_direction(p, p->bwd, 1, x, p->nrows > 0 ? p->nrows : 1, y,
+                         p->ncols > 0 ? p->ncols : 1, alpha, beta, stream);
+}
+
+int hm_matmat_t(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx, void *Y, int64_t ldy,
+                double alpha, double beta, void *stream) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    if (!p) return fail(HM_EINVAL, "plan is NULL");
+    return run_direction(p, p->bwd, nrhs, X, ldx, Y, ldy, alpha, beta, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+"""Device-resident H-matrix and its products (the drop-in for ``H*y``).
+
+``DeviceHmat`` owns one plan of libhmat_amd.so: the exported factors packed
+in a storage precision and uploaded once.  Products take and return torch
+tensors on the plan's device (torch is used only for device memory and the
+stream handle) or, through ``mul``/``rmul``, numpy arrays like the
+reference's ``H*y`` / ``v*H``:
+
+* ``Base.:*(H, v)``   src/harithm.jl:325-329  -> ``DeviceHmat.mul`` (1-D)
+* ``Base.:*(H, X)``   src/harithm.jl:317-323  -> ``DeviceHmat.mul`` (2-D)
+* ``hmat_matvec!``    src/harithm.jl:340-354  -> ``DeviceHmat.matvec(alpha=s, beta=1)``
+* ``Base.:*(v, H)``   src/harithm.jl:331-337  -> ``DeviceHmat.rmul``
+* ``transpose_hmat_mat_mul`` harithm.jl:243-260 -> ``DeviceHmat.matmat_t``
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _native
+from .flat import FlatHmat, flatten
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: H-matrix products run only on the HIP device")
+    return torch
+
+
+def _stream_handle(stream) -> ctypes.c_void_p:
+    torch = _torch()
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class DeviceHmat:
+    """One H-matrix resident on the GPU in a given storage precision."""
+
+    def __init__(self, flat: FlatHmat, dtype: str = "f64"):
+        torch = _torch()
+        if dtype not in _native.STORAGE:
+            raise ValueError(f"dtype must be one of {sorted(_native.STORAGE)}")
+        L = _native.lib()
+        torch.cuda.init()
+        self.flat = flat
+        self.dtype = dtype
+        self.vec_dtype = torch.float64 if dtype == "f64" else torch.float32
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        b = np.ascontiguousarray(flat.blocks, dtype=np.int64)
+        d = np.ascontiguousarray(flat.data, dtype=np.float64)
+        h = ctypes.c_void_p()
+        rc = L.hm_plan_create(flat.nrows, flat.ncols, b.shape[0],
+                              b.ctypes.data_as(ctypes.c_void_p), d.ctypes.data_as(ctypes.c_void_p),
+                              d.size, _native.STORAGE[dtype], ctypes.byref(h))
+        _native.check(rc, "hm_plan_create")
+        self._h = h
+        self.nrows, self.ncols = flat.nrows, flat.ncols
+
+    @classmethod
+    def from_hmat(cls, H, dtype: str = "f64") -> "DeviceHmat":
+        return cls(flatten(H), dtype)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _native.lib().hm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        out = (ctypes.c_int64 * len(_native.INFO_FIELDS))()
+        _native.check(_native.lib().hm_plan_info(self._h, out, len(out)), "hm_plan_info")
+        return dict(zip(_native.INFO_FIELDS, list(out)))
+
+    # ------------------------------------------------------------------
+    def _vec(self, t, n: int, name: str):
+        torch = _torch()
+        if not isinstance(t, torch.Tensor) or t.device != self.device or t.dtype != self.vec_dtype:
+            raise TypeError(f"{name} must be a {self.vec_dtype} tensor on {self.device}")
+        if t.dim() != 1 or t.shape[0] != n or (n > 1 and t.stride(0) != 1):
+            raise ValueError(f"{name} must be a contiguous vector of length {n}")
+        return ctypes.c_void_p(t.data_ptr())
+
+    def _mat(self, t, n: int, name: str):
+        """Column-major (n x nrhs) tensor -> (pointer, nrhs, ld)."""
+        torch = _torch()
+        if not isinstance(t, torch.Tensor) or t.device != self.device or t.dtype != self.vec_dtype:
+            raise TypeError(f"{name} must be a {self.vec_dtype} tensor on {self.device}")
+        if t.dim() != 2 or t.shape[0] != n:
+            raise ValueError(f"{name} must have {n} rows")
+        if t.shape[1] > 1 and t.stride(0) != 1:
+            raise ValueError(f"{name} must be column-major (stride(0) == 1)")
+        ld = t.stride(1) if t.shape[1] > 1 else max(1, n)
+        return ctypes.c_void_p(t.data_ptr()), t.shape[1], ld
+
+    def new_cols(self, n: int, nrhs: int):
+        """Zero column-major (n x nrhs) tensor on the device."""
+        torch = _torch()
+        return torch.zeros((nrhs, n), dtype=self.vec_dtype, device=self.device).t()
+
+    def matvec(self, x, y=None, alpha: float = 1.0, beta: float = 0.0, stream=None, transpose=False):
+        """y = beta*y + alpha*H*x (or H' with transpose=True), device tensors."""
+        torch = _torch()
+        m, n = (self.ncols, self.nrows) if transpose else (self.nrows, self.ncols)
+        px = self._vec(x, n, "x")
+        if y is None:
+            y = torch.zeros(m, dtype=self.vec_dtype, device=self.device)
+            beta = 0.0
+        py = self._vec(y, m, "y")
+        fn = "hm_matvec_t" if transpose else "hm_matvec"
+        rc = getattr(_native.lib(), fn)(self._h, px, py, float(alpha), float(beta), _stream_handle(stream))
+        _native.check(rc, fn)
+        return y
+
+    def matmat(self, X, Y=None, alpha: float = 1.0, beta: float = 0.0, stream=None, transpose=False):
+        """Y = beta*Y + alpha*H*X for column-major device matrices."""
+        m, n = (self.ncols, self.nrows) if transpose else (self.nrows, self.ncols)
+        px, nrhs, ldx = self._mat(X, n, "X")
+        if Y is None:
+            Y = self.new_cols(m, nrhs)
+            beta = 0.0
+        py, nrhs2, ldy = self._mat(Y, m, "Y")
+        if nrhs2 != nrhs:
+            raise ValueError("X and Y have different column counts")
+        fn = "hm_matmat_t" if transpose else "hm_matmat"
+        rc = getattr(_native.lib(), fn)(self._h, nrhs, px, ldx, py, ldy, float(alpha), float(beta),
+                                        _stream_handle(stream))
+        _native.check(rc, fn)
+        return Y
+
+    def matmat_t(self, X, Y=None, alpha: float = 1.0, beta: float = 0.0, stream=None):
+        return self.matmat(X, Y, alpha, beta, stream, transpose=True)
+
+    # ------------------------------------------------------------------
+    # numpy in, numpy out (the reference's H*y / H*X / v*H)
+    def mul(self, v: np.ndarray) -> np.ndarray:
+        torch = _torch()
+        v = np.asarray(v)
+        if v.ndim == 1:
+            if v.shape[0] != self.ncols:
+                raise ValueError(f"H is {self.nrows}x{self.ncols}; vector has {v.shape[0]} entries")
+            x = torch.from_numpy(np.ascontiguousarray(v)).to(self.device, self.vec_dtype)
+            return self.matvec(x).cpu().numpy().astype(np.float64)
+        if v.ndim == 2:
+            if v.shape[0] != self.ncols:
+                raise ValueError(f"H is {self.nrows}x{self.ncols}; matrix has {v.shape[0]} rows")
+            X = self.new_cols(self.ncols, v.shape[1])
+            X.copy_(torch.from_numpy(np.asarray(v)).to(self.device, self.vec_dtype))
+            return self.matmat(X).cpu().numpy().astype(np.float64)
+        raise ValueError("H * v needs a vector or a matrix")
+
+    def rmul(self, v: np.ndarray) -> np.ndarray:
+        """v (p x m) * H -> p x n."""
+        torch = _torch()
+        v = np.asarray(v)
+        if v.ndim != 2 or v.shape[1] != self.nrows:
+            raise ValueError(f"v * H needs a p x {self.nrows} array")
+        X = self.new_cols(self.nrows, v.shape[0])
+        X.copy_(torch.from_numpy(np.ascontiguousarray(v.T)).to(self.device, self.vec_dtype))
+        return self.matmat_t(X).cpu().numpy().T.astype(np.float64)
+
+
+class TransposedHmat:
+    """``H.T``: products with the transpose, ``H.T * y``."""
+
+    __array_ufunc__ = None
+
+    def __init__(self, H):
+        self.H = H
+
+    @property
+    def shape(self):
+        return (self.H.n, self.H.m)
+
+    def __mul__(self, v):
+        v = np.asarray(v, dtype=np.float64)
+        d = self.H.device("f64")
+        if v.ndim == 1:
+            return d.rmul(v[None, :])[0]
+        return d.rmul(v.T).T
+
+    __matmul__ = __mul__

@@ -1,0 +1,129 @@
+/*
+ * hmat_amd.h -- C ABI of the MI355X H-matrix product library (libhmat_amd.so).
+ *
+ * The library evaluates products with a hierarchical matrix whose leaves were
+ * built and compressed on the host (FP64) and handed over as a block table
+ * plus one FP64 factor arena ("exported factors").  Every entry point takes
+ * plain pointers and sizes, returns an int status (HM_OK == 0, negative on
+ * error, message in hm_last_error()), never aborts and never throws.
+ *
+ * Block table: int64 row-major array of nblocks rows x 8 columns
+ *   { kind, row0, col0, m, n, k, offA, offB }
+ *   kind 0 = dense leaf C (m x n, column-major at data[offA]);
+ *   kind 1 = low-rank leaf A*B' with A (m x k) at data[offA] and B (n x k) at
+ *            data[offB], both column-major.
+ *   row0/col0 are 0-based positions of the leaf in cluster order; the leaves
+ *   must tile the nrows x ncols matrix exactly once.
+ * This is the data of the reference's Hmat leaves (src/cluster.jl:21-34:
+ * C, A, B, m, n) with the tree replaced by explicit offsets.
+ *
+ * Vectors are device pointers.  Their element type follows the plan's
+ * storage: double for HM_F64 plans, float for every other storage.
+ * Products accumulate in double (HM_F64) or float (all others).
+ *
+ * Threading: one plan may be used from one stream at a time (its scratch
+ * buffers are per plan); distinct plans are independent.  A plan lives on
+ * the device that was current at hm_plan_create.
+ */
+#ifndef HMAT_AMD_H
+#define HMAT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hm_plan_s *hm_plan_t;
+
+/* storage precision of leaves and factors on the device */
+enum hm_storage {
+    HM_F64 = 0,    /* double storage, double vectors and accumulation          */
+    HM_F32 = 1,    /* float storage, float vectors, float accumulation         */
+    HM_BF16 = 2,   /* bfloat16 storage (round to nearest even), float vectors  */
+    HM_BF16X2 = 3  /* two bfloat16 planes hi + lo (value = hi + lo), float vec */
+};
+
+enum hm_status {
+    HM_OK = 0,
+    HM_EINVAL = -1,       /* bad argument, inconsistent block table          */
+    HM_EHIP = -2,         /* HIP runtime error                               */
+    HM_ENOMEM = -3,       /* device allocation failed                         */
+    HM_EUNSUPPORTED = -4  /* valid request this build does not implement      */
+};
+
+/* fields reported by hm_plan_info, in this order */
+enum hm_info_field {
+    HM_INFO_NROWS = 0,
+    HM_INFO_NCOLS,
+    HM_INFO_NDENSE,          /* dense leaves                                   */
+    HM_INFO_NRK,             /* low-rank leaves                                */
+    HM_INFO_DENSE_ELEMS,     /* sum m*n over dense leaves                      */
+    HM_INFO_RK_ELEMS,        /* sum (m+n)*k over low-rank leaves               */
+    HM_INFO_STORAGE,         /* enum hm_storage                                */
+    HM_INFO_DEVICE_BYTES,    /* device memory held by the plan                 */
+    HM_INFO_FACTOR_BYTES,    /* bytes of leaves+factors one product streams   */
+    HM_INFO_MAXRANK,
+    HM_INFO_NFIELDS
+};
+
+/*
+ * Create a plan: validate the block table, pack the factors into the device
+ * layout of the requested storage and upload them.  The host arrays are only
+ * read during the call.
+ * Replaces: the Hmat tree handed to hmat_matvec! (src/harithm.jl:340) as
+ * built by NewHmat (src/cluster.jl:36-43).
+ */
+int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks,
+                   const int64_t *blocks, const double *data, int64_t ndata,
+                   int storage, hm_plan_t *plan);
+
+/* Release a plan and its device memory.  NULL is accepted. */
+int hm_plan_destroy(hm_plan_t plan);
+
+/* Copy min(nout, HM_INFO_NFIELDS) fields of enum hm_info_field into out. */
+int hm_plan_info(hm_plan_t plan, int64_t *out, int nout);
+
+/*
+ * y = beta*y + alpha*H*x.  x has ncols entries, y nrows; both device
+ * pointers on the plan's device; x and y must not overlap.  beta == 0 does
+ * not read y.  stream: hipStream_t or NULL for the null stream.
+ * Replaces: hmat_matvec!(r, H, v, s) == (alpha = s, beta = 1)
+ * (src/harithm.jl:340-354) and Base.:*(H, v) == (alpha = 1, beta = 0)
+ * (src/harithm.jl:325-329).
+ */
+int hm_matvec(hm_plan_t plan, const void *x, void *y,
+              double alpha, double beta, void *stream);
+
+/*
+ * Y = beta*Y + alpha*H*X for nrhs right-hand sides, column-major X
+ * (ncols x nrhs, leading dimension ldx) and Y (nrows x nrhs, ldy).
+ * Replaces: Base.:*(H, X::Matrix) (src/harithm.jl:317-323), which loops
+ * hmat_matvec! over the columns.
+ */
+int hm_matmat(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx,
+              void *Y, int64_t ldy, double alpha, double beta, void *stream);
+
+/*
+ * y = beta*y + alpha*H'*x (x has nrows entries, y ncols).
+ * Replaces: hmat_matvec2 (src/harithm.jl:357-371) behind v*H
+ * (src/harithm.jl:331-337) and transpose_hmat_mat_mul (src/harithm.jl:243-260).
+ */
+int hm_matvec_t(hm_plan_t plan, const void *x, void *y,
+                double alpha, double beta, void *stream);
+
+/* Y = beta*Y + alpha*H'*X, column-major, as hm_matmat. */
+int hm_matmat_t(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx,
+                void *Y, int64_t ldy, double alpha, double beta, void *stream);
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char *hm_last_error(void);
+
+/* Library version string. */
+const char *hm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMAT_AMD_H */

@@ -1,0 +1,72 @@
+"""GPU parity of the FP64 plain product against the FP64 oracle (<= 1e-13 relative)."""
+import numpy as np
+import pytest
+
+from tests.helpers import c1, build, cauchy, expk, sorted_uniform, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-13
+
+
+@pytest.fixture(scope="module")
+def c1_aca(hm):
+    par, c, H = c1(hm, "aca2")
+    return H, hm.flatten(H)
+
+
+def test_c1_matvec_f64(hm, orc, c1_aca):
+    H, f = c1_aca
+    y = np.random.default_rng(0).random(f.ncols)
+    got = H * y
+    assert relerr(got, orc.matvec(f, y)) <= TOL64
+
+
+def test_c1_alpha_beta_accumulate(hm, orc, c1_aca):
+    import torch
+    H, f = c1_aca
+    d = H.device("f64")
+    rng = np.random.default_rng(1)
+    x, r0 = rng.standard_normal(f.ncols), rng.standard_normal(f.nrows)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.from_numpy(r0).cuda()
+    d.matvec(xd, yd, alpha=-1.0, beta=1.0)          # hmat_matvec!(r, H, v, -1.0)
+    ref = orc.matvec(f, x, r=r0, s=-1.0)
+    assert relerr(yd.cpu().numpy(), ref) <= TOL64
+    yd2 = torch.full((f.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    d.matvec(xd, yd2, alpha=2.5, beta=0.0)           # beta = 0 never reads y
+    assert relerr(yd2.cpu().numpy(), orc.matvec(f, x, s=2.5)) <= TOL64
+
+
+def test_c1_transpose_f64(hm, orc, c1_aca):
+    H, f = c1_aca
+    v = np.random.default_rng(2).standard_normal((3, f.nrows))
+    got = v * H
+    for i in range(3):
+        assert relerr(got[i], orc.matvec_t(f, v[i])) <= TOL64
+
+
+def test_c1_matmat_f64(hm, orc, c1_aca):
+    H, f = c1_aca
+    X = np.random.default_rng(3).standard_normal((f.ncols, 5))
+    assert relerr(H * X, orc.matmat(f, X)) <= TOL64
+
+
+def test_c2_scale_f64(hm, orc):
+    """N=65536 sorted uniform points on [0,100], Cauchy, MaxBlock 1024, MaxRank 16."""
+    X = sorted_uniform(65536, 0.0, 100.0, 1)
+    par, c, H = build(hm, X, cauchy, 1024, 64, 16)
+    f = hm.flatten(H)
+    y = np.random.default_rng(2).standard_normal(f.ncols)
+    got = H * y
+    assert relerr(got, orc.matvec(f, y)) <= TOL64
+
+
+@pytest.mark.parametrize("dtype,tol", [("f32", 1e-6), ("bf16x2", 2e-5), ("bf16", 1e-2)])
+def test_c1_storage_dtypes(hm, orc, c1_aca, dtype, tol):
+    import torch
+    H, f = c1_aca
+    d = hm.DeviceHmat(f, dtype)
+    x = np.random.default_rng(4).standard_normal(f.ncols)
+    y = d.matvec(torch.from_numpy(x).float().cuda()).cpu().numpy()
+    assert relerr(y, orc.matvec(f, x)) <= tol
