@@ -1,0 +1,217 @@
+"""H-matrix product benchmark on one MI355X.
+
+Workload (default): N = 262144 sorted uniform points on [0, 400] (seed 3),
+kernel exp(-|x-y|), MaxRank 24, MaxBlock N/4, MinBlock 64, weak
+admissibility (Adj = identity), ACA construction on the host in FP64;
+X = 32 standard-normal right-hand sides (seed 4).
+
+One step = one product Y = H*X of the 32 right-hand sides, inputs resident
+in HBM.  The batched path streams the factors once per step in the chosen
+storage (default: split BF16, FP32 accumulation).  The plain FP64 path
+(one matvec per right-hand side, the reference's H*X semantics) is timed
+beside it.  Several plan replicas are rotated so the working set exceeds the
+256 MB Infinity Cache.
+
+Prints one JSON line (see the repository's measurement notes for fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(hm, N: int, nrhs: int):
+    import scipy.sparse as sp
+    par = hm.Params()
+    X = np.sort(np.random.default_rng(3).uniform(0.0, 400.0, N))
+    par.Geom = X
+    par.Kernel = lambda x, y: np.exp(-np.abs(x - y))
+    par.MaxRank, par.MaxBlock, par.MinBlock = 24, -1, 64
+    par.ConsMethod = "aca2"
+    par.Adj = sp.identity(N, format="csr")
+    t0 = time.time()
+    c, H = hm.NewHmat(par)
+    f = hm.flatten(H)
+    log(f"construction N={N}: {time.time() - t0:.1f}s, {f.nblocks} leaves {f.counts()}")
+    RHS = np.random.default_rng(4).standard_normal((N, nrhs))
+    return par, c, f, RHS
+
+
+def algorithmic_bytes(f, storage: str, nrhs: int, vec_bytes: int) -> int:
+    """b_f * (sum_dense m n + sum_rk (m+n) k) + (b_x + b_y) N nrhs."""
+    b = f.blocks
+    dense = b[:, 0] == 0
+    de = int((b[dense, 3] * b[dense, 4]).sum())
+    re = int(((b[~dense, 3] + b[~dense, 4]) * b[~dense, 5]).sum())
+    fb = {"f64": 8, "f32": 4, "bf16": 2, "bf16x2": 4}[storage]
+    return fb * (de + re) + 2 * vec_bytes * f.nrows * nrhs
+
+
+def dense_rows_reference(par, c, RHS, rows):
+    """FP64 dense kernel rows: (K(x_rows, x) @ RHS) in cluster order."""
+    Xc = np.asarray(c.X)
+    out = np.zeros((rows.size, RHS.shape[1]))
+    for s in range(0, rows.size, 128):
+        r = rows[s:s + 128]
+        K = par.Kernel(Xc[r][:, None], Xc[None, :])
+        out[s:s + 128] = K @ RHS
+    return out
+
+
+def timed(torch, fn, steps: int, warmup: int, stream):
+    for i in range(warmup):
+        fn(i)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(steps):
+        fn(i)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps   # ms per step
+
+
+def cpu_baseline(f, seconds: float):
+    """Oracle FP64 matvec, 1 thread, on the same factors; bounded sample."""
+    from oracle import oracle
+    x = np.random.default_rng(9).standard_normal(f.ncols)
+    oracle.matvec(f, x)        # warm
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        oracle.matvec(f, x)
+        n += 1
+    dt = time.perf_counter() - t0
+    return n / dt, n, dt
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--nrhs", type=int, default=32)
+    ap.add_argument("--storage", default="bf16x2", choices=["f32", "bf16", "bf16x2"])
+    ap.add_argument("--replicas", type=int, default=4)
+    ap.add_argument("--plain-steps", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--sample-rows", type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the product is "
+              "not sharded across GPUs in this build", file=sys.stderr)
+        return 2
+
+    import torch
+    import __graft_entry__
+    hm = __graft_entry__.package()
+    N, nrhs = args.n, args.nrhs
+    par, c, f, RHS = build_workload(hm, N, nrhs)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+
+    # batched low-precision path, rotated replicas
+    t0 = time.time()
+    plans = [hm.DeviceHmat(f, args.storage) for _ in range(args.replicas)]
+    log(f"plans ({args.storage} x{args.replicas}) created in {time.time() - t0:.1f}s; "
+        f"info {plans[0].info()}")
+    Xs = [plans[0].new_cols(N, nrhs) for _ in range(args.replicas)]
+    Ys = [plans[0].new_cols(N, nrhs) for _ in range(args.replicas)]
+    for Xd in Xs:
+        Xd.copy_(torch.from_numpy(RHS).to(dev, torch.float32))
+    R = args.replicas
+
+    def step_batched(i):
+        plans[i % R].matmat(Xs[i % R], Ys[i % R], stream=stream)
+
+    ms = timed(torch, step_batched, args.steps, args.warmup, stream)
+    log(f"batched {args.storage}: {ms * 1e3:.1f} us per product of {nrhs} rhs")
+
+    # accuracy of the batched result
+    Yb = Ys[0].cpu().numpy().astype(np.float64)
+    rows = np.sort(np.random.default_rng(5).choice(N, size=min(args.sample_rows, N), replace=False))
+    t0 = time.time()
+    Yd = dense_rows_reference(par, c, RHS, rows)
+    relerr_dense = float(np.linalg.norm(Yb[rows] - Yd) / np.linalg.norm(Yd))
+    log(f"dense FP64 sample ({rows.size} rows) in {time.time() - t0:.1f}s; relerr {relerr_dense:.3e}")
+
+    # plain FP64 path: one matvec per right-hand side
+    p64 = hm.DeviceHmat(f, "f64")
+    X64 = p64.new_cols(N, nrhs)
+    X64.copy_(torch.from_numpy(RHS).to(dev))
+    Y64 = p64.new_cols(N, nrhs)
+
+    def step_plain(i):
+        p64.matmat(X64, Y64, stream=stream)
+
+    ms64 = timed(torch, step_plain, args.plain_steps, 1, stream)
+    Yp = Y64.cpu().numpy()
+    relerr_h64 = float(np.linalg.norm(Yb - Yp) / np.linalg.norm(Yp))
+    relerr_dense64 = float(np.linalg.norm(Yp[rows] - Yd) / np.linalg.norm(Yd))
+    log(f"plain f64: {ms64 * 1e3:.1f} us per {nrhs} rhs; relerr vs dense {relerr_dense64:.3e}; "
+        f"batched vs plain {relerr_h64:.3e}")
+
+    bytes_b = algorithmic_bytes(f, args.storage, nrhs, 4)
+    bytes_64 = algorithmic_bytes(f, "f64", 1, 8)
+    achieved = bytes_b / (ms * 1e-3) / 1e9
+    value = nrhs / (ms * 1e-3)
+    plain_mv_s = nrhs / (ms64 * 1e-3)
+    cpu = None
+    if args.cpu_seconds > 0:
+        rate, n, dt = cpu_baseline(f, args.cpu_seconds)
+        cpu = {"value": rate, "unit": "H-matvec/s", "cores": 1, "kind": "port",
+               "sample": f"{n} FP64 oracle matvecs (1 RHS each) on the same factors in {dt:.1f}s"}
+        log(f"cpu baseline: {rate:.3f} matvec/s")
+    line = {
+        "metric": "H-matvec/s (N=262144, 32 RHS, BF16-split batched, relerr<=1e-5)",
+        "value": value,
+        "unit": "H-matvec/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16x2": "bf16x2-storage/f32-acc", "bf16": "bf16-storage/f32-acc",
+                  "f32": "f32"}[args.storage],
+        "data": "synthetic",
+        "config": {"workload": f"H*X exp(-|x-y|) N={N} nrhs={nrhs} MaxRank=24 MinBlock=64 "
+                               f"MaxBlock=N/4 weak-adm, {args.storage} storage",
+                   "N": N, "nrhs": nrhs, "leaves": f.nblocks, "dense_leaves": f.counts()[0],
+                   "rk_leaves": f.counts()[1], "replicas": R},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "relerr_vs_dense_fp64": relerr_dense,
+        "relerr_vs_fp64_hmatrix": relerr_h64,
+        "plain_fp64": {"H-matvec/s": plain_mv_s, "us_per_matvec": ms64 * 1e3 / nrhs,
+                       "GB/s": bytes_64 / (ms64 * 1e-3 / nrhs) / 1e9,
+                       "relerr_vs_dense_fp64": relerr_dense64},
+        "speedup_vs_plain_fp64": value / plain_mv_s,
+        "algorithmic_bytes_per_step": bytes_b,
+    }
+    print(json.dumps(line), flush=True)
+    for p in plans + [p64]:
+        p.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

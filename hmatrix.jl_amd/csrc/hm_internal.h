@@ -66,6 +66,73 @@ struct TileItem {
     int32_t ro;       // first row of the tile inside the leaf
 };
 
+// ---------------------------------------------------------------------------
+// Batched multi-right-hand-side path (FP32 / BF16 / split-BF16 storage).
+// Right-hand sides are processed in panels of NB columns.
+constexpr int NB = 32;
+// Columns of X one projection workgroup stages in LDS.
+constexpr int PROJ_SUPER = 256;
+
+// Row-owner item of the batched path.
+struct BItem {
+    int64_t off;    // dense: fragment base (elements); rk: U slice base (elements)
+    int64_t aux;    // dense: col0; rk: t offset (floats, layout [j][NB])
+    int32_t kind;
+    int32_t dim;    // dense: n; rk: k
+};
+
+struct BTile {
+    int64_t row0;
+    int32_t rows;
+    int32_t ib, ie;
+    int32_t pad;
+};
+
+// Projection super-tile: columns [col0, col0 + rows) of X staged in LDS.
+struct PTile {
+    int64_t col0;
+    int32_t rows;
+    int32_t ib, ie;
+    int32_t pad;
+};
+
+// Projection item: partial^T = X[slice]^T V[slice] for one low-rank leaf.
+// The V slice covers super-tile rows [ls, ls + nrow) (ls and nrow multiples
+// of 8, zero rows where the leaf does not reach), column-major, ld nrow.
+struct PItem {
+    int64_t voff;   // V slice base (elements)
+    int64_t out;    // float offset of the k x NB result ([j][NB])
+    int32_t ls;
+    int32_t nrow;
+    int32_t k;
+    int32_t pad;
+};
+
+// Batched reduction: t[tOff + e] = sum_c part[pOff + c*len + e], e < len.
+struct BRed {
+    int64_t pOff;
+    int64_t tOff;
+    int32_t nchunk;
+    int32_t len;
+};
+
+struct Batched {
+    bool ready = false;
+    BTile *tiles = nullptr;
+    int64_t ntiles = 0;
+    BItem *items = nullptr;
+    PTile *ptiles = nullptr;
+    int64_t nptiles = 0;
+    PItem *pitems = nullptr;
+    BRed *red = nullptr;
+    int64_t nred = 0;
+    void *frag = nullptr;    // dense fragments
+    void *uslc = nullptr;    // U row slices per tile ([j][R4] column-major)
+    void *vslc = nullptr;    // V slices per super-tile
+    float *t = nullptr;      // [t (tLen) | partials (partLen)]
+    int64_t tLen = 0, partLen = 0;
+};
+
 // Device-side description of one product direction (H or H').
 struct Direction {
     int64_t nrows = 0, ncols = 0;
@@ -84,6 +151,7 @@ struct Direction {
     void *t = nullptr;     // scratch [t (tLen) | partials (partLen)]
     const void *denseA = nullptr;   // arena for dense leaves (fwd: fac; bwd: transposed copy)
     const void *denseA_lo = nullptr;
+    Batched bat;
 };
 
 struct Plan {
@@ -103,5 +171,8 @@ struct Plan {
 // Launchers (hm_kernels.hip).  Vectors are double for HM_F64, float otherwise.
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y,
                          double alpha, double beta, hipStream_t s);
+// Batched: one panel of ncol <= NB right-hand sides, float vectors.
+hipError_t launch_batched(const Plan &p, const Direction &d, int ncol, const float *X, int64_t ldx,
+                          float *Y, int64_t ldy, float alpha, float beta, hipStream_t s);
 
 }  // namespace hm

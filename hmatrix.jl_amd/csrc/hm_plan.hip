@@ -9,7 +9,10 @@
 //   * rows are cut at every leaf boundary and then into tiles of <= 64 rows;
 //     every tile lists the leaves that cross it, in block-table order;
 //   * low-rank leaves get projection items of <= PROJ_CHUNK rows and
-//     <= PROJ_KMAX rank columns.
+//     <= PROJ_KMAX rank columns;
+//   * FP32/BF16/split-BF16 plans also get the batched multi-right-hand-side
+//     layout (hm_batched.hip): dense leaves in MFMA fragment order per row
+//     tile, A rows per tile, B slices per 256-column super-tile.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -114,6 +117,230 @@ void pack_storage(int st, const double *src, int64_t n, std::vector<uint8_t> &hi
     }
 }
 
+
+// Row tiles: cut at every leaf row boundary, then into <= TILE_ROWS pieces.
+std::vector<std::pair<int64_t, int32_t>> make_tiles(int64_t nrows, const std::vector<OB> &obs) {
+    std::vector<int64_t> cuts;
+    cuts.reserve(2 * obs.size() + 2);
+    cuts.push_back(0);
+    cuts.push_back(nrows);
+    for (const OB &o : obs) {
+        cuts.push_back(o.row0);
+        cuts.push_back(o.row0 + o.m);
+    }
+    std::sort(cuts.begin(), cuts.end());
+    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+    std::vector<std::pair<int64_t, int32_t>> tiles;
+    for (size_t i = 0; i + 1 < cuts.size(); ++i)
+        for (int64_t r = cuts[i]; r < cuts[i + 1]; r += hm::TILE_ROWS)
+            tiles.emplace_back(r, static_cast<int32_t>(std::min<int64_t>(hm::TILE_ROWS, cuts[i + 1] - r)));
+    return tiles;
+}
+
+bool contributes(const OB &o) { return o.m > 0 && o.n > 0 && (o.kind == hm::KIND_DENSE || o.k > 0); }
+
+// Scale each low-rank column pair (A[:, l], B[:, l]) by (2^e, 2^-e) so that
+// both columns have the same magnitude; exact in FP64, it keeps FP32/BF16
+// copies of factors such as exp(-|x-y|) cross approximations (A ~ e^{+100},
+// B ~ e^{-200}) inside the representable range.
+void balance_factors(const std::vector<OB> &obs, double *d) {
+    for (const OB &o : obs) {
+        if (o.kind != hm::KIND_RK) continue;
+        for (int64_t l = 0; l < o.k; ++l) {
+            double *A = d + o.offA + l * o.m;
+            double *B = d + o.offB + l * o.n;
+            double ma = 0.0, mb = 0.0;
+            for (int64_t i = 0; i < o.m; ++i) ma = std::max(ma, std::fabs(A[i]));
+            for (int64_t j = 0; j < o.n; ++j) mb = std::max(mb, std::fabs(B[j]));
+            if (!(ma > 0.0) || !(mb > 0.0) || !std::isfinite(ma) || !std::isfinite(mb)) continue;
+            const int e = static_cast<int>(std::lround(0.5 * std::log2(mb / ma)));
+            if (e == 0) continue;
+            for (int64_t i = 0; i < o.m; ++i) A[i] = std::ldexp(A[i], e);
+            for (int64_t j = 0; j < o.n; ++j) B[j] = std::ldexp(B[j], -e);
+        }
+    }
+}
+
+// Append v converted to the batched element type (float, or bf16 planes).
+struct Arena {
+    int mode = 0;                 // 1 f32, 2 bf16, 3 bf16x2 (fragments only)
+    std::vector<float> f;
+    std::vector<uint16_t> h;
+    int64_t size() const { return mode == 1 ? static_cast<int64_t>(f.size()) : static_cast<int64_t>(h.size()); }
+    void align(int64_t a) {
+        while (size() % a) push(0.0);
+    }
+    void push(double v) {
+        if (mode == 1) f.push_back(static_cast<float>(v));
+        else h.push_back(f32_to_bf16_rne(static_cast<float>(v)));
+    }
+    void push_lo(double v) {       // low half of the split of v
+        const uint16_t hi = f32_to_bf16_rne(static_cast<float>(v));
+        h.push_back(f32_to_bf16_rne(static_cast<float>(v - bf16_to_f32_host(hi))));
+    }
+};
+
+template <class T>
+void *upload_vec(hm::Plan &p, const std::vector<T> &v) {
+    return v.empty() ? nullptr : static_cast<void *>(dev_upload(p, v));
+}
+
+// Batched (multi-RHS) layout of one direction.  rk factors come from `fac`
+// (balanced FP64 arena), dense leaves from `dense` at each leaf's offA.
+void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
+                   const std::vector<OB> &obs, const double *fac, const double *dense) {
+    using namespace hm;
+    const int mode = p.storage;           // 1, 2 or 3
+    Batched &B = d.bat;
+    // t offsets
+    std::vector<int64_t> tOff(obs.size(), -1);
+    int64_t tLen = 0;
+    for (size_t b = 0; b < obs.size(); ++b)
+        if (obs[b].kind == KIND_RK && contributes(obs[b])) {
+            tOff[b] = tLen;
+            tLen += obs[b].k * NB;
+        }
+    // ---- row owner: tiles, items, dense fragments, U slices
+    const auto tl = make_tiles(nrows, obs);
+    std::vector<int64_t> tileStart(tl.size());
+    for (size_t i = 0; i < tl.size(); ++i) tileStart[i] = tl[i].first;
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> per(tl.size());   // (block, ro)
+    for (size_t b = 0; b < obs.size(); ++b) {
+        const OB &o = obs[b];
+        if (!contributes(o)) continue;
+        size_t t = static_cast<size_t>(std::lower_bound(tileStart.begin(), tileStart.end(), o.row0) - tileStart.begin());
+        for (; t < tl.size() && tl[t].first < o.row0 + o.m; ++t)
+            per[t].emplace_back(static_cast<int32_t>(b), static_cast<int32_t>(tl[t].first - o.row0));
+    }
+    Arena fr;
+    fr.mode = mode == 1 ? 1 : 2;
+    Arena us;
+    us.mode = mode == 2 ? 2 : 1;
+    std::vector<BTile> tiles(tl.size());
+    std::vector<BItem> items;
+    for (size_t t = 0; t < tl.size(); ++t) {
+        const int32_t R = tl[t].second;
+        tiles[t].row0 = tl[t].first;
+        tiles[t].rows = R;
+        tiles[t].ib = static_cast<int32_t>(items.size());
+        for (const auto &pr : per[t]) {
+            const OB &o = obs[pr.first];
+            const int64_t ro = pr.second;
+            BItem it{};
+            it.kind = o.kind;
+            if (o.kind == KIND_DENSE) {
+                fr.align(8);
+                it.off = fr.size();
+                it.aux = o.col0;
+                it.dim = static_cast<int32_t>(o.n);
+                const double *C = dense + o.offA;
+                const int nK = static_cast<int>((o.n + 31) / 32);
+                for (int s = 0; s < nK; ++s) {
+                    const int64_t rem = o.n - 32 * s;
+                    const int nOct = rem >= 32 ? 4 : static_cast<int>((rem + 7) / 8);
+                    for (int mt = 0; mt * 16 < R; ++mt) {
+                        const int Rm = std::min(16, R - 16 * mt);
+                        for (int plane = 0; plane < (mode == 3 ? 2 : 1); ++plane)
+                            for (int g = 0; g < nOct; ++g)
+                                for (int r = 0; r < Rm; ++r)
+                                    for (int e = 0; e < 8; ++e) {
+                                        const int64_t kk = 32 * s + 8 * g + e;
+                                        const double v = kk < o.n ? C[(ro + 16 * mt + r) + kk * o.m] : 0.0;
+                                        if (plane == 0) fr.push(v);
+                                        else fr.push_lo(v);
+                                    }
+                    }
+                }
+            } else {
+                us.align(8);
+                it.off = us.size();
+                it.aux = tOff[pr.first];
+                it.dim = static_cast<int32_t>(o.k);
+                const int R4 = (R + 3) & ~3;
+                const double *A = fac + o.offA;
+                for (int64_t j = 0; j < o.k; ++j)
+                    for (int r = 0; r < R4; ++r) us.push(r < R ? A[(ro + r) + j * o.m] : 0.0);
+            }
+            items.push_back(it);
+        }
+        tiles[t].ie = static_cast<int32_t>(items.size());
+    }
+    // ---- projection: super-tiles of PROJ_SUPER columns
+    const int64_t nst = (ncols + PROJ_SUPER - 1) / PROJ_SUPER;
+    std::vector<std::vector<int32_t>> perst(static_cast<size_t>(nst));
+    for (size_t b = 0; b < obs.size(); ++b) {
+        const OB &o = obs[b];
+        if (o.kind != KIND_RK || !contributes(o)) continue;
+        for (int64_t st = o.col0 / PROJ_SUPER; st <= (o.col0 + o.n - 1) / PROJ_SUPER; ++st)
+            perst[st].push_back(static_cast<int32_t>(b));
+    }
+    std::vector<int64_t> pOff(obs.size(), -1), nch(obs.size(), 0), chunkIdx(obs.size(), 0);
+    int64_t partLen = 0;
+    std::vector<BRed> red;
+    for (size_t b = 0; b < obs.size(); ++b) {
+        const OB &o = obs[b];
+        if (o.kind != KIND_RK || !contributes(o)) continue;
+        nch[b] = (o.col0 + o.n - 1) / PROJ_SUPER - o.col0 / PROJ_SUPER + 1;
+        if (nch[b] > 1) {
+            pOff[b] = partLen;
+            partLen += nch[b] * o.k * NB;
+            red.push_back(BRed{pOff[b], tOff[b], static_cast<int32_t>(nch[b]), static_cast<int32_t>(o.k * NB)});
+        }
+    }
+    Arena vs;
+    vs.mode = mode == 2 ? 2 : 1;
+    std::vector<PTile> pt;
+    std::vector<PItem> pitems;
+    for (int64_t st = 0; st < nst; ++st) {
+        if (perst[st].empty()) continue;
+        PTile P{};
+        P.col0 = st * PROJ_SUPER;
+        P.rows = static_cast<int32_t>(std::min<int64_t>(PROJ_SUPER, ncols - P.col0));
+        P.ib = static_cast<int32_t>(pitems.size());
+        for (int32_t b : perst[st]) {
+            const OB &o = obs[b];
+            const int64_t a = std::max(o.col0, P.col0) - P.col0;
+            const int64_t e = std::min(o.col0 + o.n, P.col0 + P.rows) - P.col0;
+            const int64_t ls = a & ~int64_t(7);
+            const int64_t nrow = ((e + 7) & ~int64_t(7)) - ls;
+            PItem it{};
+            vs.align(8);
+            it.voff = vs.size();
+            it.ls = static_cast<int32_t>(ls);
+            it.nrow = static_cast<int32_t>(nrow);
+            it.k = static_cast<int32_t>(o.k);
+            if (nch[b] > 1) it.out = tLen + pOff[b] + (chunkIdx[b]++) * o.k * NB;
+            else it.out = tOff[b];
+            const double *Bf = fac + o.offB;
+            for (int64_t j = 0; j < o.k; ++j)
+                for (int64_t r = 0; r < nrow; ++r) {
+                    const int64_t i = P.col0 + ls + r - o.col0;
+                    vs.push((i >= 0 && i < o.n) ? Bf[i + j * o.n] : 0.0);
+                }
+            pitems.push_back(it);
+        }
+        P.ie = static_cast<int32_t>(pitems.size());
+        pt.push_back(P);
+    }
+    if (tl.size() >= (size_t(1) << 31) || items.size() >= (size_t(1) << 31) || pitems.size() >= (size_t(1) << 31))
+        throw std::length_error("work lists exceed 2^31 entries");
+    B.tiles = dev_upload(p, tiles);
+    B.ntiles = static_cast<int64_t>(tiles.size());
+    B.items = dev_upload(p, items);
+    B.ptiles = dev_upload(p, pt);
+    B.nptiles = static_cast<int64_t>(pt.size());
+    B.pitems = dev_upload(p, pitems);
+    B.red = dev_upload(p, red);
+    B.nred = static_cast<int64_t>(red.size());
+    B.frag = fr.mode == 1 ? upload_vec(p, fr.f) : upload_vec(p, fr.h);
+    B.uslc = us.mode == 1 ? upload_vec(p, us.f) : upload_vec(p, us.h);
+    B.vslc = vs.mode == 1 ? upload_vec(p, vs.f) : upload_vec(p, vs.h);
+    B.tLen = tLen;
+    B.partLen = partLen;
+    B.t = static_cast<float *>(dev_alloc(p, static_cast<size_t>(tLen + partLen) * sizeof(float)));
+    B.ready = true;
+}
+
 void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
                      const std::vector<OB> &obs) {
     using namespace hm;
@@ -169,25 +396,12 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
             }
         }
     }
-    // row tiles: cut at every leaf row boundary, then into <= TILE_ROWS pieces
-    std::vector<int64_t> cuts;
-    cuts.reserve(2 * obs.size() + 2);
-    cuts.push_back(0);
-    cuts.push_back(nrows);
-    for (const OB &o : obs) {
-        cuts.push_back(o.row0);
-        cuts.push_back(o.row0 + o.m);
-    }
-    std::sort(cuts.begin(), cuts.end());
-    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
     std::vector<TileDev> tiles;
-    for (size_t i = 0; i + 1 < cuts.size(); ++i) {
-        for (int64_t r = cuts[i]; r < cuts[i + 1]; r += TILE_ROWS) {
-            TileDev t{};
-            t.row0 = r;
-            t.rows = static_cast<int32_t>(std::min<int64_t>(TILE_ROWS, cuts[i + 1] - r));
-            tiles.push_back(t);
-        }
+    for (const auto &tr : make_tiles(nrows, obs)) {
+        TileDev t{};
+        t.row0 = tr.first;
+        t.rows = tr.second;
+        tiles.push_back(t);
     }
     // tile -> contributing leaves (CSR), leaves in table order
     std::vector<int64_t> tileStart(tiles.size());
@@ -196,9 +410,6 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
     auto first_tile = [&](int64_t row) {
         return static_cast<size_t>(std::lower_bound(tileStart.begin(), tileStart.end(), row) -
                                    tileStart.begin());
-    };
-    auto contributes = [](const OB &o) {
-        return o.m > 0 && o.n > 0 && (o.kind == KIND_DENSE || o.k > 0);
     };
     for (const OB &o : obs) {
         if (!contributes(o)) continue;
@@ -279,6 +490,17 @@ int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void 
     DeviceGuard g(p->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t es = acc_size(p->storage);
+    if (p->storage != HM_F64 && nrhs >= 2 && d.bat.ready) {
+        for (int64_t c0 = 0; c0 < nrhs; c0 += hm::NB) {
+            const int nc = static_cast<int>(std::min<int64_t>(hm::NB, nrhs - c0));
+            hipError_t e = hm::launch_batched(*p, d, nc, static_cast<const float *>(X) + c0 * ldx, ldx,
+                                              static_cast<float *>(Y) + c0 * ldy, ldy,
+                                              static_cast<float>(alpha), static_cast<float>(beta), s);
+            if (e != hipSuccess)
+                return fail(HM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        }
+        return HM_OK;
+    }
     for (int64_t c = 0; c < nrhs; ++c) {
         const void *xc = static_cast<const char *>(X) + c * ldx * es;
         void *yc = static_cast<char *>(Y) + c * ldy * es;
@@ -377,9 +599,11 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
     p->rk_elems = rk_elems;
     p->maxrank = maxrank;
     try {
-        // forward arena
+        // forward arena, with low-rank column pairs balanced (exact in FP64)
+        std::vector<double> bal(data, data + ndata);
+        balance_factors(fw, bal.data());
         std::vector<uint8_t> hi, lo;
-        pack_storage(storage, data, ndata, hi, lo);
+        pack_storage(storage, bal.data(), ndata, hi, lo);
         p->fac = dev_upload(*p, hi);
         p->fac_lo = lo.empty() ? nullptr : dev_upload(*p, lo);
         // transposed dense leaves, packed compactly
@@ -409,6 +633,10 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
         build_direction(*p, p->bwd, ncols, nrows, bw);
         p->bwd.denseA = p->facT;
         p->bwd.denseA_lo = p->facT_lo;
+        if (storage != HM_F64) {
+            build_batched(*p, p->fwd, nrows, ncols, fw, bal.data(), bal.data());
+            build_batched(*p, p->bwd, ncols, nrows, bw, bal.data(), dT.data());
+        }
     } catch (const HipFail &h) {
         const int code = (h.e == hipErrorOutOfMemory || h.e == hipErrorMemoryAllocation) ? HM_ENOMEM : HM_EHIP;
         std::string msg = h.where + ": " + hipGetErrorString(h.e);

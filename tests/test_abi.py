@@ -1,0 +1,55 @@
+"""The C-ABI library loads, exports every symbol include/hmat_amd.h declares,
+and rejects malformed block tables before touching the GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "hmat_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(hm_\w+)\s*\(", txt, re.M)))
+
+
+def test_exports_match_header(hm):
+    from hmatrix_amd import _native
+    L = _native.lib()
+    names = _declared()
+    assert len(names) >= 9
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _native.SIGNATURES, n
+    assert L.hm_version().decode().startswith("hmat_amd")
+
+
+def _create(blocks, data, nrows, ncols, storage=0):
+    from hmatrix_amd import _native
+    L = _native.lib()
+    b = np.ascontiguousarray(np.asarray(blocks, dtype=np.int64).reshape(-1, 8))
+    d = np.ascontiguousarray(np.asarray(data, dtype=np.float64))
+    h = ctypes.c_void_p()
+    rc = L.hm_plan_create(nrows, ncols, b.shape[0], b.ctypes.data_as(ctypes.c_void_p),
+                          d.ctypes.data_as(ctypes.c_void_p), d.size, storage, ctypes.byref(h))
+    return rc, L.hm_last_error().decode(), h
+
+
+@pytest.mark.parametrize("blocks,data,shape,msg", [
+    ([[2, 0, 0, 4, 4, 0, 0, 0]], np.zeros(16), (4, 4), "unknown kind"),
+    ([[0, 0, 0, 5, 4, 0, 0, 0]], np.zeros(20), (4, 4), "outside"),
+    ([[0, 0, 0, 4, 4, 0, 1, 0]], np.zeros(16), (4, 4), "arena"),
+    ([[1, 0, 0, 4, 4, 2, 0, 5]], np.zeros(12), (4, 4), "arena"),
+    ([[0, 0, 0, 4, 2, 0, 0, 0]], np.zeros(16), (4, 4), "tile"),
+    ([[0, 0, 0, 2, 4, 0, 0, 0], [0, 0, 0, 2, 4, 0, 0, 0]], np.zeros(8), (4, 4), "covered"),
+])
+def test_plan_create_rejects(hm, blocks, data, shape, msg):
+    rc, err, h = _create(blocks, data, *shape)
+    assert rc == -1 and msg in err and not h.value
+
+
+def test_plan_create_rejects_bad_storage(hm):
+    rc, err, h = _create([[0, 0, 0, 1, 1, 0, 0, 0]], [1.0], 1, 1, storage=9)
+    assert rc == -1 and "storage" in err

@@ -1,0 +1,82 @@
+"""GPU parity of the batched multi-right-hand-side path (MFMA) against the FP64 oracle.
+
+Tolerances are relative Frobenius norms of the error over all right-hand sides:
+split BF16 and FP32 storage (3 BF16 MFMA terms, FP32 accumulation) <= 2e-5,
+plain BF16 storage <= 1e-2.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import build, c1, cauchy, expk, sorted_uniform, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"bf16x2": 2e-5, "f32": 2e-5, "bf16": 1e-2}
+
+
+def _run(hm, f, dtype, X, transpose=False):
+    import torch
+    d = hm.DeviceHmat(f, dtype)
+    n_in = f.nrows if transpose else f.ncols
+    Xd = d.new_cols(n_in, X.shape[1])
+    Xd.copy_(torch.from_numpy(X).float())
+    Y = d.matmat(Xd, transpose=transpose)
+    out = Y.cpu().numpy().astype(np.float64)
+    d.close()
+    return out
+
+
+@pytest.fixture(scope="module")
+def c1f(hm):
+    par, c, H = c1(hm, "aca2")
+    return hm.flatten(H)
+
+
+@pytest.mark.parametrize("dtype", ["bf16x2", "f32", "bf16"])
+@pytest.mark.parametrize("nrhs", [32, 5, 40])
+def test_c1_batched(hm, orc, c1f, dtype, nrhs):
+    X = np.random.default_rng(nrhs).standard_normal((c1f.ncols, nrhs))
+    assert relerr(_run(hm, c1f, dtype, X), orc.matmat(c1f, X)) <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", ["bf16x2", "bf16"])
+def test_c1_batched_transpose(hm, orc, c1f, dtype):
+    X = np.random.default_rng(7).standard_normal((c1f.nrows, 32))
+    ref = np.stack([orc.matvec_t(c1f, X[:, i]) for i in range(32)], 1)
+    assert relerr(_run(hm, c1f, dtype, X, transpose=True), ref) <= TOL[dtype]
+
+
+def test_batched_alpha_beta(hm, orc, c1f):
+    import torch
+    d = hm.DeviceHmat(c1f, "bf16x2")
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((c1f.ncols, 32))
+    Y0 = rng.standard_normal((c1f.nrows, 32))
+    Xd = d.new_cols(c1f.ncols, 32)
+    Xd.copy_(torch.from_numpy(X).float())
+    Yd = d.new_cols(c1f.nrows, 32)
+    Yd.copy_(torch.from_numpy(Y0).float())
+    d.matmat(Xd, Yd, alpha=-1.5, beta=0.5)
+    ref = 0.5 * Y0 - 1.5 * orc.matmat(c1f, X)
+    assert relerr(Yd.cpu().numpy(), ref) <= 2e-5
+
+
+def test_exp_kernel_large_factors(hm, orc):
+    """exp(-|x-y|) on a wide interval: ACA factors span e^{+-100}; the library
+    balances them so that FP32/BF16 copies stay finite."""
+    X = sorted_uniform(16384, 0.0, 400.0, 3)
+    par, c, H = build(hm, X, expk, 4096, 64, 24)
+    f = hm.flatten(H)
+    R = np.random.default_rng(4).standard_normal((f.ncols, 32))
+    got = _run(hm, f, "bf16x2", R)
+    assert np.isfinite(got).all()
+    assert relerr(got, orc.matmat(f, R)) <= 2e-5
+
+
+def test_ragged_small(hm, orc):
+    """N smaller than one projection super-tile, uneven leaves, strong admissibility."""
+    X = sorted_uniform(300, 0.0, 7.0, 12)
+    par, c, H = build(hm, X, cauchy, 100, 16, 6, adj="strong")
+    f = hm.flatten(H)
+    R = np.random.default_rng(5).standard_normal((f.ncols, 3))
+    assert relerr(_run(hm, f, "bf16x2", R), orc.matmat(f, R)) <= 2e-5
