@@ -20,12 +20,12 @@ __all__ = [
     "Hparams", "Params", "Cluster", "Hmat", "NewHmat", "FullMat", "kernel_full",
     "construct_cluster", "bisect_cluster", "aca2", "compress", "rank_truncate", "bbfmm",
     "FlatHmat", "flatten", "to_dense", "to_fmat", "Array", "info", "save_factors",
-    "load_factors", "KIND_DENSE", "KIND_RK", "DeviceHmat",
+    "load_factors", "KIND_DENSE", "KIND_RK", "DeviceHmat", "hmatvec",
 ]
 
 
 def __getattr__(name):
-    if name in ("DeviceHmat", "TransposedHmat"):
+    if name in ("DeviceHmat", "TransposedHmat", "hmatvec"):
         from . import device
         return getattr(device, name)
     raise AttributeError(name)

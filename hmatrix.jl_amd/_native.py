@@ -30,6 +30,7 @@ SIGNATURES = {
     "hm_matmat": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _D, _D, _P]),
     "hm_matvec_t": (ctypes.c_int, [_P, _P, _P, _D, _D, _P]),
     "hm_matmat_t": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _D, _D, _P]),
+    "hm_adjoint": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, ctypes.POINTER(_D), _P]),
     "hm_last_error": (ctypes.c_char_p, []),
     "hm_version": (ctypes.c_char_p, []),
 }

@@ -14,7 +14,7 @@ enum : int { KIND_DENSE = 0, KIND_RK = 1 };
 // Rows of one work tile in the row-owner phase: one wavefront lane per row.
 constexpr int TILE_ROWS = 64;
 // Rows of V (or A for the transposed product) one projection item streams.
-constexpr int PROJ_CHUNK = 2048;
+constexpr int PROJ_CHUNK = 1024;
 // Rank columns one projection item accumulates in registers.
 constexpr int PROJ_KMAX = 16;
 // Waves per workgroup in the generic kernels.
@@ -53,17 +53,16 @@ struct RedItem {
     int32_t kc;
 };
 
-// Row tile of the owner phase.
+// Row tile of the owner phase.  The tile's work is a stream of columns
+// [c0, c1): column c holds `rows` values at stream[base + (c - c0)*rows] and
+// is multiplied by mult(c) = x[id] (id >= 0, dense leaf column) or
+// t[~id] (id < 0, low-rank column).
 struct TileDev {
     int64_t row0;
+    int64_t base;
     int32_t rows;
-    int32_t ib, ie;   // item range in the item list
+    int32_t c0, c1;
     int32_t pad;
-};
-
-struct TileItem {
-    int32_t blk;
-    int32_t ro;       // first row of the tile inside the leaf
 };
 
 // ---------------------------------------------------------------------------
@@ -144,14 +143,25 @@ struct Direction {
     int64_t nred = 0;
     TileDev *tiles = nullptr;
     int64_t ntiles = 0;
-    TileItem *items = nullptr;
-    int64_t nitems = 0;
+    int32_t *mult = nullptr;          // multiplier index per stream column
+    int64_t ncolumns = 0;
+    void *stream = nullptr;           // packed columns (storage type; hi plane)
+    void *stream_lo = nullptr;        // lo plane (BF16X2)
     int64_t tLen = 0;      // length of t (one right-hand side)
     int64_t partLen = 0;   // length of the chunk partials
     void *t = nullptr;     // scratch [t (tLen) | partials (partLen)]
-    const void *denseA = nullptr;   // arena for dense leaves (fwd: fac; bwd: transposed copy)
-    const void *denseA_lo = nullptr;
     Batched bat;
+};
+
+// Gradient item: elements [e0, e1) of one gradient block in the exported
+// arena layout.  kind 0: dC (m x n), 1: dA (m x k), 2: dB (n x k).
+struct GradItem {
+    int64_t dst;      // offset of the block in the gradient arena
+    int64_t vec0;     // first index into the vector of the block's rows (dC, dA: res; dB: x)
+    int64_t aux;      // dC: col0 (index into x); dA/dB: t offset
+    int64_t e0, e1;
+    int32_t ld;       // rows of the block (m, or n for dB)
+    int32_t kind;
 };
 
 struct Plan {
@@ -162,15 +172,23 @@ struct Plan {
     int64_t device_bytes = 0;
     void *fac = nullptr;      // storage arena (hi plane for BF16X2)
     void *fac_lo = nullptr;   // lo plane for BF16X2
-    void *facT = nullptr;     // transposed dense leaves (hi)
-    void *facT_lo = nullptr;
     Direction fwd, bwd;
+    // adjoint: gradient items and the balance exponent of every rank column
+    GradItem *grad = nullptr;
+    int64_t ngrad = 0;
+    int32_t *rexp = nullptr;       // length fwd.tLen
+    void *lossbuf = nullptr;       // one double
+    int64_t ndata = 0;
     std::vector<void *> allocs;
 };
 
 // Launchers (hm_kernels.hip).  Vectors are double for HM_F64, float otherwise.
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y,
                          double alpha, double beta, hipStream_t s);
+// Adjoint: gradients from res, x and the t vectors the two products left in
+// fwd.t (t = B' x) and bwd.t (s = A' res); optional loss into p.lossbuf.
+hipError_t launch_adjoint_grads(const Plan &p, const void *x, const void *res, void *grad,
+                                bool want_loss, hipStream_t s);
 // Batched: one panel of ncol <= NB right-hand sides, float vectors.
 hipError_t launch_batched(const Plan &p, const Direction &d, int ncol, const float *X, int64_t ldx,
                           float *Y, int64_t ldy, float alpha, float beta, hipStream_t s);

@@ -7,11 +7,12 @@
 //      chunks whose partial sums a tiny reduction kernel adds in a fixed
 //      order (deterministic, no atomics);
 //   2. row owner: one workgroup per tile of <= 64 rows of y; each lane owns
-//      one row and walks every leaf that intersects the tile: dense C rows
-//      (C column-major, so a column of the tile is one contiguous 512-byte
-//      read per wavefront for FP64) and low-rank A rows times t_b.  The 4
-//      wavefronts split the tile's leaves and combine in LDS; exactly one
-//      thread writes each y element.
+//      one row.  Every leaf that crosses the tile contributes columns: a
+//      dense leaf its n columns (multiplier x[col]), a low-rank leaf the k
+//      columns of A (multiplier t_b[j]).  The plan packs these columns
+//      tile-major, so a tile is one contiguous stream; the 4 wavefronts
+//      split it and combine in LDS; exactly one thread writes each y
+//      element.
 // This is the arithmetic of hmat_matvec! (src/harithm.jl:340-354): dense
 // leaves r += s*C*v (line 342), low-rank leaves r += s*A*(B'*v) (line 344).
 #include "hm_internal.h"
@@ -56,8 +57,21 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Broadcast lane `src` (wave-uniform) of v to the whole wavefront.
+__device__ __forceinline__ float bcast(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), src);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                static_cast<unsigned int>(lo));
+}
+
 // Pass 1: projection partials.  One wavefront per item; lanes stride the
-// rows of B, each lane keeps up to PROJ_KMAX rank accumulators.
+// rows of B, four 64-row strips in flight, each lane keeps up to PROJ_KMAX
+// rank accumulators, one wave reduction per rank column at the end.
 template <class St>
 __global__ __launch_bounds__(256) void k_proj(const BlkDev *__restrict__ blk,
                                               const ProjItem *__restrict__ items, int64_t nitems,
@@ -77,12 +91,23 @@ __global__ __launch_bounds__(256) void k_proj(const BlkDev *__restrict__ blk,
     A acc[PROJ_KMAX];
 #pragma unroll
     for (int j = 0; j < PROJ_KMAX; ++j) acc[j] = A(0);
-    for (int r = lane; r < w.rows; r += 64) {
-        const int64_t i = w.row0 + r;
-        const A xv = xs[i];
+    for (int r0 = 0; r0 < w.rows; r0 += 256) {
+        A xv[4];
+        bool ok[4];
 #pragma unroll
-        for (int j = 0; j < PROJ_KMAX; ++j)
-            if (j < w.kc) acc[j] += St::ld(fac, fac_lo, base + i + j * ld) * xv;
+        for (int u = 0; u < 4; ++u) {
+            const int r = r0 + 64 * u + lane;
+            ok[u] = r < w.rows;
+            xv[u] = ok[u] ? xs[w.row0 + r] : A(0);
+        }
+#pragma unroll
+        for (int j = 0; j < PROJ_KMAX; ++j) {
+            if (j < w.kc) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (ok[u]) acc[j] += St::ld(fac, fac_lo, base + w.row0 + r0 + 64 * u + lane + j * ld) * xv[u];
+            }
+        }
     }
 #pragma unroll
     for (int j = 0; j < PROJ_KMAX; ++j) {
@@ -108,15 +133,15 @@ __global__ __launch_bounds__(256) void k_reduce(const RedItem *__restrict__ item
     t[r.tOff + j] = s;
 }
 
-// Pass 2: row owner.  One workgroup per tile of <= 64 rows.
+// Pass 2: row owner over the tile's column stream.  One workgroup per tile;
+// the 4 wavefronts take contiguous quarters of the columns, lane = row.
+// Multipliers are fetched 64 at a time (one per lane) and broadcast with
+// readlane; eight columns are in flight per wavefront.
 template <class St>
-__global__ __launch_bounds__(256) void k_rows(const BlkDev *__restrict__ blk,
-                                              const TileDev *__restrict__ tiles,
-                                              const TileItem *__restrict__ items,
-                                              const void *__restrict__ fac,
-                                              const void *__restrict__ fac_lo,
-                                              const void *__restrict__ dfac,
-                                              const void *__restrict__ dfac_lo,
+__global__ __launch_bounds__(256) void k_rows(const TileDev *__restrict__ tiles,
+                                              const int32_t *__restrict__ mult,
+                                              const void *__restrict__ stream,
+                                              const void *__restrict__ stream_lo,
                                               const typename St::acc *__restrict__ x,
                                               const typename St::acc *__restrict__ t,
                                               typename St::acc *__restrict__ y,
@@ -127,27 +152,38 @@ __global__ __launch_bounds__(256) void k_rows(const BlkDev *__restrict__ blk,
     const TileDev T = tiles[blockIdx.x];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const bool act = lane < T.rows;
+    const int R = T.rows;
+    const bool act = lane < R;
+    const int ncol = T.c1 - T.c0;
+    const int per = (ncol + WAVES - 1) / WAVES;
+    const int cb = T.c0 + min(ncol, wave * per);
+    const int ce = T.c0 + min(ncol, (wave + 1) * per);
     A a0 = A(0), a1 = A(0), a2 = A(0), a3 = A(0);
-    for (int q = T.ib + wave; q < T.ie; q += WAVES) {
-        const TileItem it = items[q];
-        const BlkDev b = blk[it.blk];
-        if (!act) continue;
-        const int64_t m = b.m;
-        int64_t p = b.offA + it.ro + lane;
-        if (b.kind == KIND_DENSE) {
-            const A *xs = x + b.col0;
-            int j = 0;
-            for (; j + 4 <= b.n; j += 4, p += 4 * m) {
-                a0 += St::ld(dfac, dfac_lo, p) * xs[j];
-                a1 += St::ld(dfac, dfac_lo, p + m) * xs[j + 1];
-                a2 += St::ld(dfac, dfac_lo, p + 2 * m) * xs[j + 2];
-                a3 += St::ld(dfac, dfac_lo, p + 3 * m) * xs[j + 3];
-            }
-            for (; j < b.n; ++j, p += m) a0 += St::ld(dfac, dfac_lo, p) * xs[j];
-        } else {
-            const A *ts = t + b.tOff;
-            for (int j = 0; j < b.k; ++j, p += m) a1 += St::ld(fac, fac_lo, p) * ts[j];
+    for (int c = cb; c < ce; c += 64) {
+        const int nc = min(64, ce - c);
+        A mv = A(0);
+        if (lane < nc) {
+            const int32_t id = mult[c + lane];
+            mv = id >= 0 ? x[id] : t[~id];
+        }
+        const int64_t p0 = T.base + static_cast<int64_t>(c - T.c0) * R + lane;
+        int j = 0;
+        for (; j + 8 <= nc; j += 8) {
+            A v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = act ? St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j + u) * R) : A(0);
+            a0 += v[0] * bcast(mv, j);
+            a1 += v[1] * bcast(mv, j + 1);
+            a2 += v[2] * bcast(mv, j + 2);
+            a3 += v[3] * bcast(mv, j + 3);
+            a0 += v[4] * bcast(mv, j + 4);
+            a1 += v[5] * bcast(mv, j + 5);
+            a2 += v[6] * bcast(mv, j + 6);
+            a3 += v[7] * bcast(mv, j + 7);
+        }
+        for (; j < nc; ++j) {
+            const A v = act ? St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j) * R) : A(0);
+            a0 += v * bcast(mv, j);
         }
     }
     red[wave][lane] = (a0 + a1) + (a2 + a3);
@@ -159,6 +195,80 @@ __global__ __launch_bounds__(256) void k_rows(const BlkDev *__restrict__ blk,
         const int64_t r = T.row0 + lane;
         y[r] = beta_zero ? alpha * s : beta * y[r] + alpha * s;
     }
+}
+
+// Adjoint gradients of L = 0.5*||H x - b||^2 in the exported arena layout:
+// dC = res x', dA = res t' and dB = x s', with t = B'x and s = A'res taken
+// from the projection passes of the two products (scaled back by the
+// power-of-two balance of each rank column, exactly).  One workgroup per
+// item of <= GRAD_CHUNK contiguous gradient elements: every write is
+// coalesced.
+template <class A>
+__global__ __launch_bounds__(256) void k_grad(const GradItem *__restrict__ items,
+                                              const int32_t *__restrict__ rexp,
+                                              const A *__restrict__ x, const A *__restrict__ res,
+                                              const A *__restrict__ t, const A *__restrict__ s,
+                                              A *__restrict__ grad) {
+    const GradItem it = items[blockIdx.x];
+    for (int64_t e = it.e0 + threadIdx.x; e < it.e1; e += 256) {
+        const int64_t i = e % it.ld;
+        const int64_t j = e / it.ld;
+        A v;
+        if (it.kind == 0) {
+            v = res[it.vec0 + i] * x[it.aux + j];
+        } else if (it.kind == 1) {
+            const int64_t q = it.aux + j;
+            v = res[it.vec0 + i] * static_cast<A>(ldexp(static_cast<double>(t[q]), rexp[q]));
+        } else {
+            const int64_t q = it.aux + j;
+            v = x[it.vec0 + i] * static_cast<A>(ldexp(static_cast<double>(s[q]), -rexp[q]));
+        }
+        grad[it.dst + e] = v;
+    }
+}
+
+// 0.5 * sum(res^2) into *out (double), one workgroup, fixed order.
+template <class A>
+__global__ __launch_bounds__(256) void k_loss(const A *__restrict__ res, int64_t n,
+                                              double *__restrict__ out) {
+    __shared__ double part[256];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const double v = static_cast<double>(res[i]);
+        acc += v * v;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = 0.5 * part[0];
+}
+
+template <class A>
+static hipError_t run_grads(const Plan &p, const void *x, const void *res, void *grad,
+                            bool want_loss, hipStream_t s) {
+    if (p.ngrad > 0 && grad) {
+        hipLaunchKernelGGL(k_grad<A>, dim3(static_cast<unsigned>(p.ngrad)), dim3(256), 0, s, p.grad,
+                           p.rexp, static_cast<const A *>(x), static_cast<const A *>(res),
+                           static_cast<const A *>(p.fwd.t), static_cast<const A *>(p.bwd.t),
+                           static_cast<A *>(grad));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (want_loss) {
+        hipLaunchKernelGGL(k_loss<A>, dim3(1), dim3(256), 0, s, static_cast<const A *>(res), p.nrows,
+                           static_cast<double *>(p.lossbuf));
+        return hipGetLastError();
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_adjoint_grads(const Plan &p, const void *x, const void *res, void *grad,
+                                bool want_loss, hipStream_t s) {
+    return p.storage == 0 ? run_grads<double>(p, x, res, grad, want_loss, s)
+                          : run_grads<float>(p, x, res, grad, want_loss, s);
 }
 
 template <class St>
@@ -188,9 +298,8 @@ static hipError_t run(const Plan &p, const Direction &d, const void *xv, void *y
     }
     if (d.ntiles > 0) {
         hipLaunchKernelGGL(k_rows<St>, dim3(static_cast<unsigned>(d.ntiles)), dim3(64 * WAVES), 0,
-                           s, d.blk, d.tiles, d.items, p.fac, p.fac_lo, d.denseA, d.denseA_lo, x,
-                           static_cast<const A *>(t), y, static_cast<A>(alpha), static_cast<A>(beta),
-                           beta == 0.0 ? 1 : 0);
+                           s, d.tiles, d.mult, d.stream, d.stream_lo, x, static_cast<const A *>(t), y,
+                           static_cast<A>(alpha), static_cast<A>(beta), beta == 0.0 ? 1 : 0);
         return hipGetLastError();
     }
     return hipSuccess;

@@ -1,13 +1,14 @@
 // Plan construction and the C ABI (include/hmat_amd.h).
 //
 // hm_plan_create turns the exported block table into device work lists:
-//   * the arena is converted once to the storage precision and uploaded;
-//   * dense leaves get a second, transposed copy so that H' x runs through
-//     the same row-owner kernel as H x (the transposed product is then as
-//     bandwidth-friendly as the forward one; low-rank factors are shared,
-//     with A and B swapping roles);
+//   * low-rank factors go to one compact arena in the storage precision
+//     (column pairs balanced by powers of two), read by the projection pass;
 //   * rows are cut at every leaf boundary and then into tiles of <= 64 rows;
-//     every tile lists the leaves that cross it, in block-table order;
+//     every tile's dense columns and A columns are packed into one
+//     contiguous column stream (row-owner pass);
+//   * the transposed product H' gets the same structures built from the
+//     transposed leaves (dense leaves transposed on the host, A and B
+//     swapping roles), so H' x runs at the same bandwidth as H x;
 //   * low-rank leaves get projection items of <= PROJ_CHUNK rows and
 //     <= PROJ_KMAX rank columns;
 //   * FP32/BF16/split-BF16 plans also get the batched multi-right-hand-side
@@ -143,22 +144,28 @@ bool contributes(const OB &o) { return o.m > 0 && o.n > 0 && (o.kind == hm::KIND
 // both columns have the same magnitude; exact in FP64, it keeps FP32/BF16
 // copies of factors such as exp(-|x-y|) cross approximations (A ~ e^{+100},
 // B ~ e^{-200}) inside the representable range.
-void balance_factors(const std::vector<OB> &obs, double *d) {
+// Returns the exponent e of every rank column of the leaves with k > 0, in
+// table order (the order of the t vectors).
+std::vector<int32_t> balance_factors(const std::vector<OB> &obs, double *d) {
+    std::vector<int32_t> ex;
     for (const OB &o : obs) {
-        if (o.kind != hm::KIND_RK) continue;
+        if (o.kind != hm::KIND_RK || o.k == 0) continue;
         for (int64_t l = 0; l < o.k; ++l) {
             double *A = d + o.offA + l * o.m;
             double *B = d + o.offB + l * o.n;
             double ma = 0.0, mb = 0.0;
             for (int64_t i = 0; i < o.m; ++i) ma = std::max(ma, std::fabs(A[i]));
             for (int64_t j = 0; j < o.n; ++j) mb = std::max(mb, std::fabs(B[j]));
-            if (!(ma > 0.0) || !(mb > 0.0) || !std::isfinite(ma) || !std::isfinite(mb)) continue;
-            const int e = static_cast<int>(std::lround(0.5 * std::log2(mb / ma)));
+            int e = 0;
+            if (ma > 0.0 && mb > 0.0 && std::isfinite(ma) && std::isfinite(mb))
+                e = static_cast<int>(std::lround(0.5 * std::log2(mb / ma)));
+            ex.push_back(e);
             if (e == 0) continue;
             for (int64_t i = 0; i < o.m; ++i) A[i] = std::ldexp(A[i], e);
             for (int64_t j = 0; j < o.n; ++j) B[j] = std::ldexp(B[j], -e);
         }
     }
+    return ex;
 }
 
 // Append v converted to the batched element type (float, or bf16 planes).
@@ -341,8 +348,11 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
     B.ready = true;
 }
 
+// Single-right-hand-side layout of one direction: projection items over the
+// factor arena `fac` (uploaded once per plan) and the row tiles' column
+// streams built from `fac` (low-rank A) and `dense` (dense leaves).
 void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
-                     const std::vector<OB> &obs) {
+                     const std::vector<OB> &obs, const double *fac, const double *dense) {
     using namespace hm;
     d.nrows = nrows;
     d.ncols = ncols;
@@ -396,40 +406,56 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
             }
         }
     }
-    std::vector<TileDev> tiles;
-    for (const auto &tr : make_tiles(nrows, obs)) {
-        TileDev t{};
-        t.row0 = tr.first;
-        t.rows = tr.second;
-        tiles.push_back(t);
-    }
-    // tile -> contributing leaves (CSR), leaves in table order
-    std::vector<int64_t> tileStart(tiles.size());
-    for (size_t i = 0; i < tiles.size(); ++i) tileStart[i] = tiles[i].row0;
-    std::vector<int32_t> cnt(tiles.size() + 1, 0);
-    auto first_tile = [&](int64_t row) {
-        return static_cast<size_t>(std::lower_bound(tileStart.begin(), tileStart.end(), row) -
-                                   tileStart.begin());
-    };
-    for (const OB &o : obs) {
-        if (!contributes(o)) continue;
-        for (size_t t = first_tile(o.row0); t < tiles.size() && tiles[t].row0 < o.row0 + o.m; ++t)
-            cnt[t + 1]++;
-    }
-    for (size_t t = 0; t < tiles.size(); ++t) cnt[t + 1] += cnt[t];
-    std::vector<TileItem> items(static_cast<size_t>(cnt[tiles.size()]));
-    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    // row tiles and their column streams (tile-major, leaves in table order)
+    const auto tl = make_tiles(nrows, obs);
+    std::vector<int64_t> tileStart(tl.size());
+    for (size_t i = 0; i < tl.size(); ++i) tileStart[i] = tl[i].first;
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> per(tl.size());   // (leaf, row offset)
     for (int64_t b = 0; b < nb; ++b) {
         const OB &o = obs[b];
         if (!contributes(o)) continue;
-        for (size_t t = first_tile(o.row0); t < tiles.size() && tiles[t].row0 < o.row0 + o.m; ++t)
-            items[fill[t]++] = TileItem{static_cast<int32_t>(b),
-                                        static_cast<int32_t>(tiles[t].row0 - o.row0)};
+        size_t t = static_cast<size_t>(std::lower_bound(tileStart.begin(), tileStart.end(), o.row0) -
+                                       tileStart.begin());
+        for (; t < tl.size() && tl[t].first < o.row0 + o.m; ++t)
+            per[t].emplace_back(static_cast<int32_t>(b), static_cast<int32_t>(tl[t].first - o.row0));
     }
-    for (size_t t = 0; t < tiles.size(); ++t) {
-        tiles[t].ib = cnt[t];
-        tiles[t].ie = cnt[t + 1];
+    std::vector<TileDev> tiles(tl.size());
+    std::vector<int32_t> mult;
+    std::vector<double> col;     // FP64 stream, converted to storage below
+    for (size_t t = 0; t < tl.size(); ++t) {
+        const int64_t R = tl[t].second;
+        TileDev &T = tiles[t];
+        T.row0 = tl[t].first;
+        T.rows = static_cast<int32_t>(R);
+        T.base = static_cast<int64_t>(col.size());
+        if (mult.size() >= (size_t(1) << 31)) throw std::length_error("column stream exceeds 2^31 columns");
+        T.c0 = static_cast<int32_t>(mult.size());
+        for (const auto &pr : per[t]) {
+            const OB &o = obs[pr.first];
+            const int64_t ro = pr.second;
+            if (o.kind == KIND_DENSE) {
+                const double *C = dense + o.offA;
+                for (int64_t j = 0; j < o.n; ++j) {
+                    mult.push_back(static_cast<int32_t>(o.col0 + j));
+                    for (int64_t r = 0; r < R; ++r) col.push_back(C[(ro + r) + j * o.m]);
+                }
+            } else {
+                const double *Af = fac + o.offA;
+                for (int64_t j = 0; j < o.k; ++j) {
+                    mult.push_back(~static_cast<int32_t>(blk[pr.first].tOff + j));
+                    for (int64_t r = 0; r < R; ++r) col.push_back(Af[(ro + r) + j * o.m]);
+                }
+            }
+        }
+        T.c1 = static_cast<int32_t>(mult.size());
     }
+    if (tLen >= (int64_t(1) << 31)) throw std::length_error("rank total exceeds 2^31");
+    std::vector<uint8_t> shi, slo;
+    pack_storage(p.storage, col.data(), static_cast<int64_t>(col.size()), shi, slo);
+    d.stream = shi.empty() ? nullptr : static_cast<void *>(dev_upload(p, shi));
+    d.stream_lo = slo.empty() ? nullptr : static_cast<void *>(dev_upload(p, slo));
+    d.mult = dev_upload(p, mult);
+    d.ncolumns = static_cast<int64_t>(mult.size());
     d.blk = dev_upload(p, blk);
     d.nblk = nb;
     d.proj = dev_upload(p, proj);
@@ -438,8 +464,6 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
     d.nred = static_cast<int64_t>(red.size());
     d.tiles = dev_upload(p, tiles);
     d.ntiles = static_cast<int64_t>(tiles.size());
-    d.items = dev_upload(p, items);
-    d.nitems = static_cast<int64_t>(items.size());
     d.tLen = tLen;
     d.partLen = partLen;
     d.t = dev_alloc(p, static_cast<size_t>(tLen + partLen) * acc_size(p.storage));
@@ -599,13 +623,27 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
     p->rk_elems = rk_elems;
     p->maxrank = maxrank;
     try {
-        // forward arena, with low-rank column pairs balanced (exact in FP64)
-        std::vector<double> bal(data, data + ndata);
-        balance_factors(fw, bal.data());
+        // Low-rank factors are copied into their own compact arena (the
+        // projection kernels read it on the device), with each column pair
+        // balanced (exact in FP64).  Dense leaves stay in `data` on the host;
+        // the kernels read them from the packed streams built below.
+        std::vector<double> rkv(static_cast<size_t>(rk_elems));
+        {
+            int64_t o2 = 0;
+            for (OB &o : fw) {
+                if (o.kind != KIND_RK) continue;
+                std::memcpy(rkv.data() + o2, data + o.offA, sizeof(double) * o.m * o.k);
+                std::memcpy(rkv.data() + o2 + o.m * o.k, data + o.offB, sizeof(double) * o.n * o.k);
+                o.offA = o2;
+                o.offB = o2 + o.m * o.k;
+                o2 += (o.m + o.n) * o.k;
+            }
+        }
+        const std::vector<int32_t> rexp = balance_factors(fw, rkv.data());
         std::vector<uint8_t> hi, lo;
-        pack_storage(storage, bal.data(), ndata, hi, lo);
-        p->fac = dev_upload(*p, hi);
-        p->fac_lo = lo.empty() ? nullptr : dev_upload(*p, lo);
+        pack_storage(storage, rkv.data(), rk_elems, hi, lo);
+        p->fac = hi.empty() ? nullptr : static_cast<void *>(dev_upload(*p, hi));
+        p->fac_lo = lo.empty() ? nullptr : static_cast<void *>(dev_upload(*p, lo));
         // transposed dense leaves, packed compactly
         std::vector<double> dT(static_cast<size_t>(dense_elems));
         std::vector<OB> bw(fw.size());
@@ -623,19 +661,38 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
             }
             bw[b] = t;
         }
-        std::vector<uint8_t> thi, tlo;
-        pack_storage(storage, dT.data(), dense_elems, thi, tlo);
-        p->facT = dev_upload(*p, thi);
-        p->facT_lo = tlo.empty() ? nullptr : dev_upload(*p, tlo);
-        build_direction(*p, p->fwd, nrows, ncols, fw);
-        p->fwd.denseA = p->fac;
-        p->fwd.denseA_lo = p->fac_lo;
-        build_direction(*p, p->bwd, ncols, nrows, bw);
-        p->bwd.denseA = p->facT;
-        p->bwd.denseA_lo = p->facT_lo;
+        build_direction(*p, p->fwd, nrows, ncols, fw, rkv.data(), data);
+        build_direction(*p, p->bwd, ncols, nrows, bw, rkv.data(), dT.data());
+        // adjoint gradient items over the caller's own arena offsets
+        {
+            constexpr int64_t GRAD_CHUNK = 16384;
+            std::vector<GradItem> gi;
+            auto add = [&](int kind, int64_t dst, int64_t len, int64_t ld, int64_t vec0, int64_t aux) {
+                for (int64_t e0 = 0; e0 < len; e0 += GRAD_CHUNK)
+                    gi.push_back(GradItem{dst, vec0, aux, e0, std::min(len, e0 + GRAD_CHUNK),
+                                          static_cast<int32_t>(ld), kind});
+            };
+            int64_t tpos = 0;
+            for (int64_t b = 0; b < nblocks; ++b) {
+                const int64_t *r = blocks + 8 * b;   // caller's offsets
+                const int64_t m = r[3], n = r[4], k = r[5];
+                if (r[0] == KIND_DENSE) {
+                    if (m > 0 && n > 0) add(0, r[6], m * n, m, r[1], r[2]);
+                } else if (k > 0) {
+                    if (m > 0) add(1, r[6], m * k, m, r[1], tpos);
+                    if (n > 0) add(2, r[7], n * k, n, r[2], tpos);
+                    tpos += k;
+                }
+            }
+            p->grad = dev_upload(*p, gi);
+            p->ngrad = static_cast<int64_t>(gi.size());
+            p->rexp = dev_upload(*p, rexp);
+            p->lossbuf = dev_alloc(*p, sizeof(double));
+            p->ndata = ndata;
+        }
         if (storage != HM_F64) {
-            build_batched(*p, p->fwd, nrows, ncols, fw, bal.data(), bal.data());
-            build_batched(*p, p->bwd, ncols, nrows, bw, bal.data(), dT.data());
+            build_batched(*p, p->fwd, nrows, ncols, fw, rkv.data(), data);
+            build_batched(*p, p->bwd, ncols, nrows, bw, rkv.data(), dT.data());
         }
     } catch (const HipFail &h) {
         const int code = (h.e == hipErrorOutOfMemory || h.e == hipErrorMemoryAllocation) ? HM_ENOMEM : HM_EHIP;
@@ -701,6 +758,32 @@ int hm_matmat_t(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx, void *
     hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
     if (!p) return fail(HM_EINVAL, "plan is NULL");
     return run_direction(p, p->bwd, nrhs, X, ldx, Y, ldy, alpha, beta, stream);
+}
+
+int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx, void *grad,
+               double *loss, void *stream) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    if (!p) return fail(HM_EINVAL, "plan is NULL");
+    if ((p->ncols > 0 && (!x || !dx)) || (p->nrows > 0 && (!b || !res)))
+        return fail(HM_EINVAL, "NULL vector");
+    if (res == b || res == x || dx == x || dx == res)
+        return fail(HM_EINVAL, "res and dx must not alias the inputs or each other");
+    DeviceGuard g(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t es = acc_size(p->storage);
+    hipError_t e = hipSuccess;
+    if (p->nrows > 0) e = hipMemcpyAsync(res, b, static_cast<size_t>(p->nrows) * es, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hm::launch_matvec(*p, p->fwd, x, res, 1.0, -1.0, s);
+    if (e == hipSuccess) e = hm::launch_matvec(*p, p->bwd, res, dx, 1.0, 0.0, s);
+    if (e == hipSuccess) e = hm::launch_adjoint_grads(*p, x, res, grad, loss != nullptr, s);
+    if (e == hipSuccess && loss) {
+        double h = 0.0;
+        e = hipMemcpyAsync(&h, p->lossbuf, sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        *loss = h;
+    }
+    if (e != hipSuccess) return fail(HM_EHIP, std::string("hm_adjoint: ") + hipGetErrorString(e));
+    return HM_OK;
 }
 
 }  // extern "C"

@@ -138,6 +138,27 @@ class DeviceHmat:
     def matmat_t(self, X, Y=None, alpha: float = 1.0, beta: float = 0.0, stream=None):
         return self.matmat(X, Y, alpha, beta, stream, transpose=True)
 
+    def adjoint(self, x, b, grad: bool = True, loss: bool = True, stream=None):
+        """Gradients of L = 0.5*||H x - b||^2 on the device.
+
+        Returns ``(L, res, dx, g)``: ``res = H x - b``, ``dx = H' res`` and
+        ``g`` the factor gradients in the exported arena layout (``None``
+        when ``grad`` is False; ``L`` is ``None`` when ``loss`` is False).
+        """
+        torch = _torch()
+        px = self._vec(x, self.ncols, "x")
+        pb = self._vec(b, self.nrows, "b")
+        res = torch.empty(self.nrows, dtype=self.vec_dtype, device=self.device)
+        dx = torch.empty(self.ncols, dtype=self.vec_dtype, device=self.device)
+        g = torch.zeros(self.flat.data.size, dtype=self.vec_dtype, device=self.device) if grad else None
+        L = ctypes.c_double(0.0)
+        rc = _native.lib().hm_adjoint(self._h, px, pb, ctypes.c_void_p(res.data_ptr()),
+                                      ctypes.c_void_p(dx.data_ptr()),
+                                      ctypes.c_void_p(g.data_ptr()) if grad else None,
+                                      ctypes.byref(L) if loss else None, _stream_handle(stream))
+        _native.check(rc, "hm_adjoint")
+        return (L.value if loss else None), res, dx, g
+
     # ------------------------------------------------------------------
     # numpy in, numpy out (the reference's H*y / H*X / v*H)
     def mul(self, v: np.ndarray) -> np.ndarray:
@@ -187,3 +208,37 @@ class TransposedHmat:
         return d.rmul(v.T).T
 
     __matmul__ = __mul__
+
+
+def hmatvec(dev: DeviceHmat, x):
+    """Differentiable ``H x`` for torch autograd (gradient with respect to x:
+    ``H' g``), H fixed."""
+    return _HMatVec.apply(x, dev)
+
+
+def _make_fn():
+    import torch
+
+    class HMatVec(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x, dev):
+            ctx.dev = dev
+            return dev.matvec(x.contiguous())
+
+        @staticmethod
+        def backward(ctx, g):
+            return ctx.dev.matvec(g.contiguous(), transpose=True), None
+
+    return HMatVec
+
+
+class _Lazy:
+    fn = None
+
+    def apply(self, *a):
+        if _Lazy.fn is None:
+            _Lazy.fn = _make_fn()
+        return _Lazy.fn.apply(*a)
+
+
+_HMatVec = _Lazy()

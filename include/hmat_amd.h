@@ -116,6 +116,24 @@ int hm_matvec_t(hm_plan_t plan, const void *x, void *y,
 int hm_matmat_t(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx,
                 void *Y, int64_t ldy, double alpha, double beta, void *stream);
 
+/*
+ * Adjoint of the least-squares loss L = 0.5*||H x - b||^2 (no counterpart in
+ * the reference; it differentiates the product of src/harithm.jl:340-354):
+ *   res  (nrows) = H x - b
+ *   dx   (ncols) = H' res                     (dL/dx)
+ *   grad (ndata) in the layout of the exported arena passed to
+ *        hm_plan_create: dense leaf C -> dC = res[rows] x[cols]' at its offA,
+ *        low-rank leaf -> dA = res[rows] (B'x)' at offA and
+ *        dB = x[cols] (A'res)' at offB; entries no leaf covers are not
+ *        written.  grad may be NULL (no factor gradients).
+ *   loss (host, may be NULL): L; when given the call waits for the stream.
+ * Vector and gradient element type is the plan's vector type (double for
+ * HM_F64, float otherwise).  All pointers except loss are device pointers;
+ * res and dx must not alias x, b or each other.
+ */
+int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx, void *grad,
+               double *loss, void *stream);
+
 /* Message of the last failing call on this host thread ("" if none). */
 const char *hm_last_error(void);
 

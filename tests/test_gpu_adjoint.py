@@ -1,0 +1,106 @@
+"""Adjoint of L = 0.5||Hx - b||^2 on the GPU: against the FP64 oracle's closed
+form (<= 1e-13), the dense A' gradient, and central finite differences with
+h = 1e-6 (FP64) evaluated through the GPU product itself.
+
+The central difference (L(p+h) - L(p-h)) / 2h is evaluated in the identical
+form (r+ - r-).(r+ + r-) / 4h, with r+- the GPU residuals at p +- h: at
+N = 65536 L is ~1e7, and subtracting the two losses directly would lose
+~1e-3 of the difference to rounding."""
+import numpy as np
+import pytest
+
+from tests.helpers import build, c1, cauchy, sorted_uniform, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(hm, f):
+    return hm.DeviceHmat(f, "f64")
+
+
+def _cd(rp, rm, h):
+    """(L(p+h) - L(p-h)) / 2h from the two residual vectors."""
+    rp, rm = rp.cpu().numpy(), rm.cpu().numpy()
+    return float((rp - rm) @ (rp + rm)) / (4 * h)
+
+
+@pytest.fixture(scope="module")
+def c5(hm):
+    """N=65536 sorted uniform points on [0,100] (seed 7), Cauchy, MaxBlock 1024, MaxRank 16."""
+    X = sorted_uniform(65536, 0.0, 100.0, 7)
+    par, c, H = build(hm, X, cauchy, 1024, 64, 16)
+    return hm.flatten(H)
+
+
+def test_adjoint_matches_oracle_c1(hm, orc):
+    import torch
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    rng = np.random.default_rng(8)
+    x, b = rng.standard_normal(f.ncols), rng.standard_normal(f.nrows)
+    d = _dev(hm, f)
+    L, res, dx, g = d.adjoint(torch.from_numpy(x).cuda(), torch.from_numpy(b).cuda())
+    Lr, resr, dxr, gr = orc.adjoint(f, x, b)
+    assert abs(L - Lr) <= 1e-13 * Lr
+    assert relerr(res.cpu().numpy(), resr) <= 1e-13
+    assert relerr(dx.cpu().numpy(), dxr) <= 1e-13
+    assert relerr(g.cpu().numpy(), gr) <= 1e-13
+
+
+def test_adjoint_c5_oracle_dense_and_fd(hm, orc, c5):
+    import torch
+    f = c5
+    rng = np.random.default_rng(8)
+    x, b = rng.standard_normal(f.ncols), rng.standard_normal(f.nrows)
+    d = _dev(hm, f)
+    xd, bd = torch.from_numpy(x).cuda(), torch.from_numpy(b).cuda()
+    L, res, dx, g = d.adjoint(xd, bd)
+    Lr, resr, dxr, gr = orc.adjoint(f, x, b)
+    dx, g = dx.cpu().numpy(), g.cpu().numpy()
+    assert relerr(dx, dxr) <= 1e-13 and relerr(g, gr) <= 1e-13
+    # dense A' gradient on sampled columns: dL/dx_j = sum_i H_ij res_i
+    cols = rng.choice(f.ncols, 64, replace=False)
+    E = np.zeros((f.ncols, 64))
+    E[cols, np.arange(64)] = 1.0
+    Hcols = orc.matmat(f, E)                       # columns of H
+    assert relerr(dx[cols], Hcols.T @ resr) <= 1e-12
+    # central differences in x through the GPU product (L is quadratic: exact up to rounding)
+    h = 1e-6
+    for j in cols[:8]:
+        e = torch.zeros_like(xd)
+        e[j] = h
+        rp = d.adjoint(xd + e, bd, grad=False, loss=False)[1]
+        rm = d.adjoint(xd - e, bd, grad=False, loss=False)[1]
+        assert abs(_cd(rp, rm, h) - dx[j]) <= 1e-6 * max(1.0, abs(dx[j]))
+    # central differences in factor entries (dense C, A, B) on sampled positions
+    blocks = f.blocks
+    picks = []
+    for kind in (0, 1):
+        idx = np.nonzero(blocks[:, 0] == kind)[0]
+        for q in rng.choice(idx, 3, replace=False):
+            kd, r0, c0, m, n, k, oa, ob = blocks[q]
+            picks.append(oa + rng.integers(0, m * (n if kd == 0 else k)))
+            if kd == 1:
+                picks.append(ob + rng.integers(0, n * k))
+    for p in picks:
+        vals = []
+        for sgn in (1, -1):
+            data = f.data.copy()
+            data[p] += sgn * h
+            dp = hm.DeviceHmat(hm.FlatHmat(f.nrows, f.ncols, f.blocks, data), "f64")
+            vals.append(dp.adjoint(xd, bd, grad=False, loss=False)[1])
+            dp.close()
+        fd = _cd(vals[0], vals[1], h)
+        assert abs(fd - g[p]) <= 1e-6 * max(1.0, abs(g[p]))
+
+
+def test_torch_autograd_x(hm, orc):
+    import torch
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    d = _dev(hm, f)
+    x = torch.from_numpy(np.random.default_rng(3).standard_normal(f.ncols)).cuda().requires_grad_(True)
+    y = hm.hmatvec(d, x)
+    w = torch.from_numpy(np.random.default_rng(4).standard_normal(f.nrows)).cuda()
+    (y * w).sum().backward()
+    assert relerr(x.grad.cpu().numpy(), orc.matvec_t(f, w.cpu().numpy())) <= 1e-13
