@@ -19,6 +19,8 @@ constexpr int PROJ_CHUNK = 1024;
 constexpr int PROJ_KMAX = 16;
 // Waves per workgroup in the generic kernels.
 constexpr int WAVES = 4;
+// Size of the plan's zero buffer (covers one t vector and any 16-byte load).
+constexpr int ZERO_BYTES = 4096;
 
 // One leaf as the kernels see it.  Offsets are element offsets into the
 // storage arena; for the transposed direction A/B and row/col are swapped
@@ -72,32 +74,75 @@ constexpr int NB = 32;
 // Columns of X one projection workgroup stages in LDS.
 constexpr int PROJ_SUPER = 256;
 
-// Row-owner item of the batched path.
-struct BItem {
-    int64_t off;    // dense: fragment base (elements); rk: U slice base (elements)
-    int64_t aux;    // dense: col0; rk: t offset (floats, layout [j][NB])
+// k-rows of B one staging batch of the row-owner kernel holds in LDS.
+constexpr int KSTAGE = 128;
+// K-steps (32 k-rows each) per batch.
+constexpr int KSLOTS = KSTAGE / 32;
+
+// K segment of a batched row tile: an R x klen block of A (a dense leaf's
+// rows, or the tile's panel of low-rank columns) whose B rows are X rows
+// [src, src + klen) (kind 0) or the t vectors tidx[src .. src + klen)
+// (kind 1).  It is staged at LDS row `lrow` of its batch.
+struct BSeg {
+    int64_t src;
+    int32_t klen;
     int32_t kind;
-    int32_t dim;    // dense: n; rk: k
+    int32_t lrow;
+    int32_t pad;
+};
+
+// One K-step (32 k values) of A fragments: all M-tiles at `frag`; B rows at
+// LDS row `krow`; nOct = number of stored 8-value octets (4, or fewer in a
+// segment's last K-step).
+struct BKs {
+    int64_t frag;
+    int32_t krow;
+    int32_t nOct;
+};
+
+// Staging batch: segments [sb, se) staged together, K-steps [kb, ke).
+struct BBatch {
+    int32_t sb, se, kb, ke;
 };
 
 struct BTile {
     int64_t row0;
     int32_t rows;
-    int32_t ib, ie;
+    int32_t bb, be;    // batch range
     int32_t pad;
 };
 
-// Projection super-tile: columns [col0, col0 + rows) of X staged in LDS.
+// Per-tile descriptor blob read by the row-owner kernel in one round trip:
+// BLOB_STRIDE 32-bit words per tile.  Layout (words):
+//   0-1 row0, 2 rows, 3 nbatch, 4 nseg, 5 nks, 6 ntoff,
+//   7 words of the full blob when it does not fit inline (0 otherwise),
+//   8-9 word offset of that full blob in the overflow arena,
+//   then from word BLOB_HDR: batches (4 words: sb, se, kb, ke; indices local
+//   to the tile), segments (6 words: src lo/hi, klen, kind, lrow, tbase),
+//   K-steps (4 words: frag lo/hi, krow, nOct), t offsets (1 word each, in
+//   units of NB floats; kind-1 segment row r uses toff[tbase + r]).
+constexpr int BLOB_STRIDE = 128;
+constexpr int BLOB_HDR = 16;
+constexpr int BLOB_MAX = 2048;      // words of the largest blob the kernel stages
+
+// Projection work unit: columns [col0, col0 + rows) of X and the B slices
+// of items [ib, ie) (bytes [vbeg, vbeg + vlen) of the slice arena, split
+// BF16 planes or FP32), all staged in LDS.  A super-tile whose slices exceed
+// the stage or PITEMS_MAX items is split into several units.
+constexpr int VSTAGE_BYTES = 32768;
+constexpr int PITEMS_MAX = 64;
 struct PTile {
     int64_t col0;
+    int64_t vbeg;
+    int32_t vlen;
     int32_t rows;
     int32_t ib, ie;
-    int32_t pad;
 };
 
 // Projection item: partial^T = X[slice]^T V[slice] for one low-rank leaf.
 // The V slice covers super-tile rows [ls, ls + nrow) (ls and nrow multiples
-// of 8, zero rows where the leaf does not reach), column-major, ld nrow.
+// of 8, zero rows where the leaf does not reach), column-major, ld nrow; for
+// split-BF16 storage the lo plane follows the hi plane (k*nrow elements).
 struct PItem {
     int64_t voff;   // V slice base (elements)
     int64_t out;    // float offset of the k x NB result ([j][NB])
@@ -117,16 +162,15 @@ struct BRed {
 
 struct Batched {
     bool ready = false;
-    BTile *tiles = nullptr;
     int64_t ntiles = 0;
-    BItem *items = nullptr;
+    uint32_t *blobs = nullptr;    // ntiles x BLOB_STRIDE words (multi-batch tiles)
+    uint32_t *blobx = nullptr;    // overflow blobs
     PTile *ptiles = nullptr;
     int64_t nptiles = 0;
     PItem *pitems = nullptr;
     BRed *red = nullptr;
     int64_t nred = 0;
-    void *frag = nullptr;    // dense fragments
-    void *uslc = nullptr;    // U row slices per tile ([j][R4] column-major)
+    void *frag = nullptr;    // A fragments of every tile segment
     void *vslc = nullptr;    // V slices per super-tile
     float *t = nullptr;      // [t (tLen) | partials (partLen)]
     int64_t tLen = 0, partLen = 0;
@@ -172,6 +216,9 @@ struct Plan {
     int64_t device_bytes = 0;
     void *fac = nullptr;      // storage arena (hi plane for BF16X2)
     void *fac_lo = nullptr;   // lo plane for BF16X2
+    // ZERO_BYTES of zeros: inactive lanes load from here instead of taking a
+    // branch, so a wavefront's loads all issue before its first wait.
+    void *zeros = nullptr;
     Direction fwd, bwd;
     // adjoint: gradient items and the balance exponent of every rank column
     GradItem *grad = nullptr;

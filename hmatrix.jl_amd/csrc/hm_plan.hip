@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -207,7 +208,7 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
             tOff[b] = tLen;
             tLen += obs[b].k * NB;
         }
-    // ---- row owner: tiles, items, dense fragments, U slices
+    // ---- row owner: tiles, K segments, A fragments
     const auto tl = make_tiles(nrows, obs);
     std::vector<int64_t> tileStart(tl.size());
     for (size_t i = 0; i < tl.size(); ++i) tileStart[i] = tl[i].first;
@@ -221,56 +222,108 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
     }
     Arena fr;
     fr.mode = mode == 1 ? 1 : 2;
-    Arena us;
-    us.mode = mode == 2 ? 2 : 1;
+    const int planes = mode == 3 ? 2 : 1;
     std::vector<BTile> tiles(tl.size());
-    std::vector<BItem> items;
+    std::vector<BBatch> batches;
+    std::vector<BSeg> segs;
+    std::vector<BKs> ksteps;
+    std::vector<int64_t> tidx;
+    // Pack the R x klen block `val(r, kk)` as MFMA A fragments: for every
+    // K-step s and M-tile mt, `planes` planes of nOct octets x Rm rows x 8.
+    auto pack = [&](int R, int64_t klen, const std::function<double(int, int64_t)> &val) {
+        fr.align(8);
+        const int64_t base = fr.size();
+        const int nK = static_cast<int>((klen + 31) / 32);
+        for (int s = 0; s < nK; ++s) {
+            const int64_t rem = klen - 32 * s;
+            const int nOct = rem >= 32 ? 4 : static_cast<int>((rem + 7) / 8);
+            for (int mt = 0; mt * 16 < R; ++mt) {
+                const int Rm = std::min(16, R - 16 * mt);
+                for (int plane = 0; plane < planes; ++plane)
+                    for (int g = 0; g < nOct; ++g)
+                        for (int r = 0; r < Rm; ++r)
+                            for (int e = 0; e < 8; ++e) {
+                                const int64_t kk = 32 * s + 8 * g + e;
+                                const double v = kk < klen ? val(16 * mt + r, kk) : 0.0;
+                                if (plane == 0) fr.push(v);
+                                else fr.push_lo(v);
+                            }
+            }
+        }
+        return base;
+    };
+    struct PendingSeg { BSeg seg; int64_t frag; };
     for (size_t t = 0; t < tl.size(); ++t) {
         const int32_t R = tl[t].second;
-        tiles[t].row0 = tl[t].first;
-        tiles[t].rows = R;
-        tiles[t].ib = static_cast<int32_t>(items.size());
+        BTile &T = tiles[t];
+        T.row0 = tl[t].first;
+        T.rows = R;
+        std::vector<PendingSeg> ps;
+        // dense leaves, cut into pieces of <= KSTAGE columns
         for (const auto &pr : per[t]) {
             const OB &o = obs[pr.first];
+            if (o.kind != KIND_DENSE) continue;
             const int64_t ro = pr.second;
-            BItem it{};
-            it.kind = o.kind;
-            if (o.kind == KIND_DENSE) {
-                fr.align(8);
-                it.off = fr.size();
-                it.aux = o.col0;
-                it.dim = static_cast<int32_t>(o.n);
-                const double *C = dense + o.offA;
-                const int nK = static_cast<int>((o.n + 31) / 32);
-                for (int s = 0; s < nK; ++s) {
-                    const int64_t rem = o.n - 32 * s;
-                    const int nOct = rem >= 32 ? 4 : static_cast<int>((rem + 7) / 8);
-                    for (int mt = 0; mt * 16 < R; ++mt) {
-                        const int Rm = std::min(16, R - 16 * mt);
-                        for (int plane = 0; plane < (mode == 3 ? 2 : 1); ++plane)
-                            for (int g = 0; g < nOct; ++g)
-                                for (int r = 0; r < Rm; ++r)
-                                    for (int e = 0; e < 8; ++e) {
-                                        const int64_t kk = 32 * s + 8 * g + e;
-                                        const double v = kk < o.n ? C[(ro + 16 * mt + r) + kk * o.m] : 0.0;
-                                        if (plane == 0) fr.push(v);
-                                        else fr.push_lo(v);
-                                    }
-                    }
-                }
-            } else {
-                us.align(8);
-                it.off = us.size();
-                it.aux = tOff[pr.first];
-                it.dim = static_cast<int32_t>(o.k);
-                const int R4 = (R + 3) & ~3;
-                const double *A = fac + o.offA;
-                for (int64_t j = 0; j < o.k; ++j)
-                    for (int r = 0; r < R4; ++r) us.push(r < R ? A[(ro + r) + j * o.m] : 0.0);
+            const double *C = dense + o.offA;
+            for (int64_t c0 = 0; c0 < o.n; c0 += KSTAGE) {
+                const int64_t kl = std::min<int64_t>(KSTAGE, o.n - c0);
+                PendingSeg q{};
+                q.seg.src = o.col0 + c0;
+                q.seg.klen = static_cast<int32_t>(kl);
+                q.seg.kind = 0;
+                q.frag = pack(R, kl, [&](int r, int64_t kk) { return C[(ro + r) + (c0 + kk) * o.m]; });
+                ps.push_back(q);
             }
-            items.push_back(it);
         }
-        tiles[t].ie = static_cast<int32_t>(items.size());
+        // low-rank leaves: the panel of their A columns, B rows = t vectors
+        std::vector<std::pair<const double *, int64_t>> ucols;   // column pointer (row ro), t offset
+        for (const auto &pr : per[t]) {
+            const OB &o = obs[pr.first];
+            if (o.kind != KIND_RK) continue;
+            for (int64_t j = 0; j < o.k; ++j)
+                ucols.emplace_back(fac + o.offA + j * o.m + pr.second, tOff[pr.first] + j * NB);
+        }
+        for (size_t c0 = 0; c0 < ucols.size(); c0 += KSTAGE) {
+            const int64_t kl = std::min<int64_t>(KSTAGE, static_cast<int64_t>(ucols.size() - c0));
+            PendingSeg q{};
+            q.seg.src = static_cast<int64_t>(tidx.size());
+            q.seg.klen = static_cast<int32_t>(kl);
+            q.seg.kind = 1;
+            for (int64_t j = 0; j < kl; ++j) tidx.push_back(ucols[c0 + j].second);
+            q.frag = pack(R, kl, [&](int r, int64_t kk) { return ucols[c0 + kk].first[r]; });
+            ps.push_back(q);
+        }
+        // batches of segments whose rounded k-rows fit the LDS stage
+        T.bb = static_cast<int32_t>(batches.size());
+        size_t i = 0;
+        while (i < ps.size()) {
+            BBatch bt{};
+            bt.sb = static_cast<int32_t>(segs.size());
+            bt.kb = static_cast<int32_t>(ksteps.size());
+            int32_t lrow = 0;
+            while (i < ps.size()) {
+                const int32_t k32 = (ps[i].seg.klen + 31) & ~31;   // K-steps start 32 rows apart
+                if (lrow + k32 > KSTAGE) break;
+                BSeg sg = ps[i].seg;
+                sg.lrow = lrow;
+                segs.push_back(sg);
+                const int nK = (sg.klen + 31) / 32;
+                for (int s = 0; s < nK; ++s) {
+                    const int rem = sg.klen - 32 * s;
+                    BKs ks{};
+                    ks.frag = ps[i].frag + static_cast<int64_t>(s) * planes * 32 * R;
+                    ks.krow = lrow + 32 * s;
+                    ks.nOct = rem >= 32 ? 4 : (rem + 7) / 8;
+                    ksteps.push_back(ks);
+                }
+                lrow += k32;
+                ++i;
+            }
+            bt.se = static_cast<int32_t>(segs.size());
+            bt.ke = static_cast<int32_t>(ksteps.size());
+            batches.push_back(bt);
+        }
+        T.be = static_cast<int32_t>(batches.size());
     }
     // ---- projection: super-tiles of PROJ_SUPER columns
     const int64_t nst = (ncols + PROJ_SUPER - 1) / PROJ_SUPER;
@@ -295,7 +348,7 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
         }
     }
     Arena vs;
-    vs.mode = mode == 2 ? 2 : 1;
+    vs.mode = mode == 1 ? 1 : 2;      // split-BF16 slices: hi plane then lo plane
     std::vector<PTile> pt;
     std::vector<PItem> pitems;
     for (int64_t st = 0; st < nst; ++st) {
@@ -303,15 +356,33 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
         PTile P{};
         P.col0 = st * PROJ_SUPER;
         P.rows = static_cast<int32_t>(std::min<int64_t>(PROJ_SUPER, ncols - P.col0));
-        P.ib = static_cast<int32_t>(pitems.size());
+        auto open_unit = [&]() {
+            vs.align(8);
+            P.ib = static_cast<int32_t>(pitems.size());
+            P.vbeg = vs.size();
+        };
+        auto close_unit = [&]() {
+            vs.align(8);
+            P.ie = static_cast<int32_t>(pitems.size());
+            P.vlen = static_cast<int32_t>(vs.size() - P.vbeg);
+            if (P.ie > P.ib) pt.push_back(P);
+        };
+        open_unit();
         for (int32_t b : perst[st]) {
             const OB &o = obs[b];
             const int64_t a = std::max(o.col0, P.col0) - P.col0;
             const int64_t e = std::min(o.col0 + o.n, P.col0 + P.rows) - P.col0;
             const int64_t ls = a & ~int64_t(7);
             const int64_t nrow = ((e + 7) & ~int64_t(7)) - ls;
-            PItem it{};
+            const int64_t esz = mode == 1 ? 4 : 2;
+            const int64_t ibytes = nrow * o.k * esz * (mode == 3 ? 2 : 1);
+            if (ibytes > VSTAGE_BYTES) throw std::length_error("rank too large for the projection stage");
             vs.align(8);
+            if ((vs.size() - P.vbeg) * esz + ibytes > VSTAGE_BYTES || pitems.size() - P.ib >= size_t(PITEMS_MAX)) {
+                close_unit();
+                open_unit();
+            }
+            PItem it{};
             it.voff = vs.size();
             it.ls = static_cast<int32_t>(ls);
             it.nrow = static_cast<int32_t>(nrow);
@@ -319,28 +390,93 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
             if (nch[b] > 1) it.out = tLen + pOff[b] + (chunkIdx[b]++) * o.k * NB;
             else it.out = tOff[b];
             const double *Bf = fac + o.offB;
-            for (int64_t j = 0; j < o.k; ++j)
-                for (int64_t r = 0; r < nrow; ++r) {
-                    const int64_t i = P.col0 + ls + r - o.col0;
-                    vs.push((i >= 0 && i < o.n) ? Bf[i + j * o.n] : 0.0);
-                }
+            for (int plane = 0; plane < (mode == 3 ? 2 : 1); ++plane)
+                for (int64_t j = 0; j < o.k; ++j)
+                    for (int64_t r = 0; r < nrow; ++r) {
+                        const int64_t i = P.col0 + ls + r - o.col0;
+                        const double v = (i >= 0 && i < o.n) ? Bf[i + j * o.n] : 0.0;
+                        if (plane == 0) vs.push(v);
+                        else vs.push_lo(v);
+                    }
             pitems.push_back(it);
         }
-        P.ie = static_cast<int32_t>(pitems.size());
-        pt.push_back(P);
+        close_unit();
     }
-    if (tl.size() >= (size_t(1) << 31) || items.size() >= (size_t(1) << 31) || pitems.size() >= (size_t(1) << 31))
+    if (tl.size() >= (size_t(1) << 31) || ksteps.size() >= (size_t(1) << 31) || pitems.size() >= (size_t(1) << 31))
         throw std::length_error("work lists exceed 2^31 entries");
-    B.tiles = dev_upload(p, tiles);
-    B.ntiles = static_cast<int64_t>(tiles.size());
-    B.items = dev_upload(p, items);
+    // per-tile descriptor blobs (inline, or header + overflow blob)
+    std::vector<uint32_t> blobs, blobx;
+    for (size_t t = 0; t < tiles.size(); ++t) {
+        const BTile &T = tiles[t];
+        std::vector<uint32_t> w(BLOB_HDR, 0u);
+        auto put64 = [&](int64_t v) {
+            w.push_back(static_cast<uint32_t>(static_cast<uint64_t>(v) & 0xffffffffu));
+            w.push_back(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
+        };
+        const int32_t sb0 = T.be > T.bb ? batches[T.bb].sb : 0;
+        const int32_t kb0 = T.be > T.bb ? batches[T.bb].kb : 0;
+        const int32_t se1 = T.be > T.bb ? batches[T.be - 1].se : 0;
+        const int32_t ke1 = T.be > T.bb ? batches[T.be - 1].ke : 0;
+        for (int32_t b = T.bb; b < T.be; ++b) {
+            w.push_back(static_cast<uint32_t>(batches[b].sb - sb0));
+            w.push_back(static_cast<uint32_t>(batches[b].se - sb0));
+            w.push_back(static_cast<uint32_t>(batches[b].kb - kb0));
+            w.push_back(static_cast<uint32_t>(batches[b].ke - kb0));
+        }
+        std::vector<uint32_t> toff;
+        for (int32_t q = sb0; q < se1; ++q) {
+            const BSeg &sg = segs[q];
+            put64(sg.src);
+            w.push_back(static_cast<uint32_t>(sg.klen));
+            w.push_back(static_cast<uint32_t>(sg.kind));
+            w.push_back(static_cast<uint32_t>(sg.lrow));
+            w.push_back(static_cast<uint32_t>(toff.size()));
+            if (sg.kind == 1)
+                for (int32_t j = 0; j < sg.klen; ++j) {
+                    const int64_t to = tidx[sg.src + j];
+                    if (to % NB || to / NB >= (int64_t(1) << 32)) throw std::length_error("t offset out of range");
+                    toff.push_back(static_cast<uint32_t>(to / NB));
+                }
+        }
+        for (int32_t k = kb0; k < ke1; ++k) {
+            put64(ksteps[k].frag);
+            w.push_back(static_cast<uint32_t>(ksteps[k].krow));
+            w.push_back(static_cast<uint32_t>(ksteps[k].nOct));
+        }
+        w.insert(w.end(), toff.begin(), toff.end());
+        while (w.size() % 4) w.push_back(0u);
+        w[0] = static_cast<uint32_t>(static_cast<uint64_t>(T.row0) & 0xffffffffu);
+        w[1] = static_cast<uint32_t>(static_cast<uint64_t>(T.row0) >> 32);
+        w[2] = static_cast<uint32_t>(T.rows);
+        w[3] = static_cast<uint32_t>(T.be - T.bb);
+        w[4] = static_cast<uint32_t>(se1 - sb0);
+        w[5] = static_cast<uint32_t>(ke1 - kb0);
+        w[6] = static_cast<uint32_t>(toff.size());
+        w.resize(std::max(w.size(), static_cast<size_t>(BLOB_STRIDE)), 0u);
+        blobs.resize(blobs.size() + BLOB_STRIDE, 0u);
+        uint32_t *dst = blobs.data() + blobs.size() - BLOB_STRIDE;
+        if (w.size() == static_cast<size_t>(BLOB_STRIDE)) {
+            std::copy(w.begin(), w.end(), dst);
+        } else {
+            if (w.size() > static_cast<size_t>(BLOB_MAX)) throw std::length_error("tile descriptor exceeds BLOB_MAX");
+            while (blobx.size() % 4) blobx.push_back(0u);
+            const int64_t at = static_cast<int64_t>(blobx.size());
+            w[7] = static_cast<uint32_t>(w.size());
+            w[8] = static_cast<uint32_t>(static_cast<uint64_t>(at) & 0xffffffffu);
+            w[9] = static_cast<uint32_t>(static_cast<uint64_t>(at) >> 32);
+            blobx.insert(blobx.end(), w.begin(), w.end());
+            std::copy(w.begin(), w.begin() + BLOB_HDR, dst);
+        }
+    }
+    B.ntiles = static_cast<int64_t>(blobs.size() / BLOB_STRIDE);
+    B.blobs = dev_upload(p, blobs);
+    B.blobx = dev_upload(p, blobx);
     B.ptiles = dev_upload(p, pt);
     B.nptiles = static_cast<int64_t>(pt.size());
     B.pitems = dev_upload(p, pitems);
     B.red = dev_upload(p, red);
     B.nred = static_cast<int64_t>(red.size());
     B.frag = fr.mode == 1 ? upload_vec(p, fr.f) : upload_vec(p, fr.h);
-    B.uslc = us.mode == 1 ? upload_vec(p, us.f) : upload_vec(p, us.h);
     B.vslc = vs.mode == 1 ? upload_vec(p, vs.f) : upload_vec(p, vs.h);
     B.tLen = tLen;
     B.partLen = partLen;
@@ -640,6 +776,7 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
             }
         }
         const std::vector<int32_t> rexp = balance_factors(fw, rkv.data());
+        p->zeros = dev_upload(*p, std::vector<uint8_t>(ZERO_BYTES, 0));
         std::vector<uint8_t> hi, lo;
         pack_storage(storage, rkv.data(), rk_elems, hi, lo);
         p->fac = hi.empty() ? nullptr : static_cast<void *>(dev_upload(*p, hi));
