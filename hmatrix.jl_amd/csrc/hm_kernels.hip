@@ -91,21 +91,25 @@ __global__ __launch_bounds__(256) void k_proj(const BlkDev *__restrict__ blk,
     A acc[PROJ_KMAX];
 #pragma unroll
     for (int j = 0; j < PROJ_KMAX; ++j) acc[j] = A(0);
+    // Lanes past the item's rows load a clamped (valid) row and multiply by
+    // zero: every load is unconditional, so all of them issue before the
+    // first wait.
     for (int r0 = 0; r0 < w.rows; r0 += 256) {
         A xv[4];
-        bool ok[4];
+        int rc[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = r0 + 64 * u + lane;
-            ok[u] = r < w.rows;
-            xv[u] = ok[u] ? xs[w.row0 + r] : A(0);
+            rc[u] = min(r, w.rows - 1);
+            const A v = xs[w.row0 + rc[u]];
+            xv[u] = r < w.rows ? v : A(0);
         }
 #pragma unroll
         for (int j = 0; j < PROJ_KMAX; ++j) {
             if (j < w.kc) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    if (ok[u]) acc[j] += St::ld(fac, fac_lo, base + w.row0 + r0 + 64 * u + lane + j * ld) * xv[u];
+                    acc[j] += St::ld(fac, fac_lo, base + w.row0 + rc[u] + j * ld) * xv[u];
             }
         }
     }
@@ -159,19 +163,19 @@ __global__ __launch_bounds__(256) void k_rows(const TileDev *__restrict__ tiles,
     const int cb = T.c0 + min(ncol, wave * per);
     const int ce = T.c0 + min(ncol, (wave + 1) * per);
     A a0 = A(0), a1 = A(0), a2 = A(0), a3 = A(0);
+    // Lanes past the tile's rows read row R-1 (a valid address) and their
+    // sums are never written: every load is unconditional.
+    const int lr = min(lane, R - 1);
     for (int c = cb; c < ce; c += 64) {
         const int nc = min(64, ce - c);
-        A mv = A(0);
-        if (lane < nc) {
-            const int32_t id = mult[c + lane];
-            mv = id >= 0 ? x[id] : t[~id];
-        }
-        const int64_t p0 = T.base + static_cast<int64_t>(c - T.c0) * R + lane;
+        const int32_t id = mult[c + min(lane, nc - 1)];
+        const A mv = *(id >= 0 ? x + id : t + ~id);
+        const int64_t p0 = T.base + static_cast<int64_t>(c - T.c0) * R + lr;
         int j = 0;
         for (; j + 8 <= nc; j += 8) {
             A v[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = act ? St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j + u) * R) : A(0);
+            for (int u = 0; u < 8; ++u) v[u] = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j + u) * R);
             a0 += v[0] * bcast(mv, j);
             a1 += v[1] * bcast(mv, j + 1);
             a2 += v[2] * bcast(mv, j + 2);
@@ -182,7 +186,7 @@ __global__ __launch_bounds__(256) void k_rows(const TileDev *__restrict__ tiles,
             a3 += v[7] * bcast(mv, j + 7);
         }
         for (; j < nc; ++j) {
-            const A v = act ? St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j) * R) : A(0);
+            const A v = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j) * R);
             a0 += v * bcast(mv, j);
         }
     }
