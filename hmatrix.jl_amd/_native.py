@@ -16,7 +16,15 @@ HM_F64, HM_F32, HM_BF16, HM_BF16X2 = 0, 1, 2, 3
 STORAGE = {"f64": HM_F64, "f32": HM_F32, "bf16": HM_BF16, "bf16x2": HM_BF16X2}
 STATUS = {0: "HM_OK", -1: "HM_EINVAL", -2: "HM_EHIP", -3: "HM_ENOMEM", -4: "HM_EUNSUPPORTED"}
 INFO_FIELDS = ("nrows", "ncols", "ndense", "nrk", "dense_elems", "rk_elems", "storage",
-               "device_bytes", "factor_bytes", "maxrank")
+               "device_bytes", "factor_bytes", "maxrank", "proj_chunk", "row_waves", "flags")
+HM_NO_TRANSPOSE, HM_NO_BATCHED = 1, 2
+HM_OP_N, HM_OP_T = 0, 1
+
+
+class HmConfig(ctypes.Structure):
+    """struct hm_config (include/hmat_amd.h)."""
+    _fields_ = [("proj_chunk", ctypes.c_int32), ("row_waves", ctypes.c_int32),
+                ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 # Every symbol include/hmat_amd.h declares, with its ctypes signature.
 _P = ctypes.c_void_p
@@ -24,6 +32,10 @@ _I64 = ctypes.c_int64
 _D = ctypes.c_double
 SIGNATURES = {
     "hm_plan_create": (ctypes.c_int, [_I64, _I64, _I64, _P, _P, _I64, ctypes.c_int, ctypes.POINTER(_P)]),
+    "hm_plan_create_ex": (ctypes.c_int, [_I64, _I64, _I64, _P, _P, _I64, ctypes.c_int, _P,
+                                         ctypes.POINTER(_P)]),
+    "hm_plan_create_batch": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P, ctypes.c_int, _P,
+                                            ctypes.POINTER(_P)]),
     "hm_plan_destroy": (ctypes.c_int, [_P]),
     "hm_plan_info": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.c_int]),
     "hm_matvec": (ctypes.c_int, [_P, _P, _P, _D, _D, _P]),
@@ -31,6 +43,10 @@ SIGNATURES = {
     "hm_matvec_t": (ctypes.c_int, [_P, _P, _P, _D, _D, _P]),
     "hm_matmat_t": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _D, _D, _P]),
     "hm_adjoint": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, ctypes.POINTER(_D), _P]),
+    "hm_graph_create": (ctypes.c_int, [_P, ctypes.c_int, _I64, _P, _I64, _P, _I64, _D, _D,
+                                       ctypes.POINTER(_P)]),
+    "hm_graph_launch": (ctypes.c_int, [_P, _P]),
+    "hm_graph_destroy": (ctypes.c_int, [_P]),
     "hm_last_error": (ctypes.c_char_p, []),
     "hm_version": (ctypes.c_char_p, []),
 }

@@ -13,7 +13,8 @@ enum : int { KIND_DENSE = 0, KIND_RK = 1 };
 
 // Rows of one work tile in the row-owner phase: one wavefront lane per row.
 constexpr int TILE_ROWS = 64;
-// Rows of V (or A for the transposed product) one projection item streams.
+// Default rows of V (or A for the transposed product) one projection item
+// streams; the plan's configuration may change it.
 constexpr int PROJ_CHUNK = 1024;
 // Rank columns one projection item accumulates in registers.
 constexpr int PROJ_KMAX = 16;
@@ -211,6 +212,9 @@ struct GradItem {
 struct Plan {
     int device = 0;
     int storage = 0;
+    int proj_chunk = PROJ_CHUNK;   // rows per projection item (single right-hand side)
+    int row_waves = WAVES;         // wavefronts per row tile (single right-hand side): 2, 4, 8
+    int flags = 0;
     int64_t nrows = 0, ncols = 0;
     int64_t ndense = 0, nrk = 0, dense_elems = 0, rk_elems = 0, maxrank = 0;
     int64_t device_bytes = 0;

@@ -138,11 +138,11 @@ __global__ __launch_bounds__(256) void k_reduce(const RedItem *__restrict__ item
 }
 
 // Pass 2: row owner over the tile's column stream.  One workgroup per tile;
-// the 4 wavefronts take contiguous quarters of the columns, lane = row.
+// its NW wavefronts take contiguous parts of the columns, lane = row.
 // Multipliers are fetched 64 at a time (one per lane) and broadcast with
 // readlane; eight columns are in flight per wavefront.
-template <class St>
-__global__ __launch_bounds__(256) void k_rows(const TileDev *__restrict__ tiles,
+template <class St, int NW>
+__global__ __launch_bounds__(64 * NW) void k_rows(const TileDev *__restrict__ tiles,
                                               const int32_t *__restrict__ mult,
                                               const void *__restrict__ stream,
                                               const void *__restrict__ stream_lo,
@@ -152,14 +152,14 @@ __global__ __launch_bounds__(256) void k_rows(const TileDev *__restrict__ tiles,
                                               typename St::acc alpha, typename St::acc beta,
                                               int beta_zero) {
     using A = typename St::acc;
-    __shared__ A red[WAVES][64];
+    __shared__ A red[NW][64];
     const TileDev T = tiles[blockIdx.x];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int R = T.rows;
     const bool act = lane < R;
     const int ncol = T.c1 - T.c0;
-    const int per = (ncol + WAVES - 1) / WAVES;
+    const int per = (ncol + NW - 1) / NW;
     const int cb = T.c0 + min(ncol, wave * per);
     const int ce = T.c0 + min(ncol, (wave + 1) * per);
     A a0 = A(0), a1 = A(0), a2 = A(0), a3 = A(0);
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_rows(const TileDev *__restrict__ tiles,
     if (wave == 0 && act) {
         A s = red[0][lane];
 #pragma unroll
-        for (int w = 1; w < WAVES; ++w) s += red[w][lane];
+        for (int w = 1; w < NW; ++w) s += red[w][lane];
         const int64_t r = T.row0 + lane;
         y[r] = beta_zero ? alpha * s : beta * y[r] + alpha * s;
     }
@@ -301,9 +301,19 @@ static hipError_t run(const Plan &p, const Direction &d, const void *xv, void *y
         }
     }
     if (d.ntiles > 0) {
-        hipLaunchKernelGGL(k_rows<St>, dim3(static_cast<unsigned>(d.ntiles)), dim3(64 * WAVES), 0,
-                           s, d.tiles, d.mult, d.stream, d.stream_lo, x, static_cast<const A *>(t), y,
-                           static_cast<A>(alpha), static_cast<A>(beta), beta == 0.0 ? 1 : 0);
+        const dim3 grid(static_cast<unsigned>(d.ntiles));
+        const A *tt = static_cast<const A *>(t);
+        const A al = static_cast<A>(alpha), be = static_cast<A>(beta);
+        const int bz = beta == 0.0 ? 1 : 0;
+        if (p.row_waves == 2)
+            hipLaunchKernelGGL((k_rows<St, 2>), grid, dim3(128), 0, s, d.tiles, d.mult, d.stream, d.stream_lo,
+                               x, tt, y, al, be, bz);
+        else if (p.row_waves == 8)
+            hipLaunchKernelGGL((k_rows<St, 8>), grid, dim3(512), 0, s, d.tiles, d.mult, d.stream, d.stream_lo,
+                               x, tt, y, al, be, bz);
+        else
+            hipLaunchKernelGGL((k_rows<St, 4>), grid, dim3(256), 0, s, d.tiles, d.mult, d.stream, d.stream_lo,
+                               x, tt, y, al, be, bz);
         return hipGetLastError();
     }
     return hipSuccess;

@@ -520,7 +520,8 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
     for (int64_t b = 0; b < nb; ++b) {
         const OB &o = obs[b];
         if (o.kind != KIND_RK || o.k == 0 || o.n == 0) continue;
-        const int64_t nchunk = (o.n + PROJ_CHUNK - 1) / PROJ_CHUNK;
+        const int64_t chunk = p.proj_chunk;
+        const int64_t nchunk = (o.n + chunk - 1) / chunk;
         for (int64_t j0 = 0; j0 < o.k; j0 += PROJ_KMAX) {
             const int32_t kc = static_cast<int32_t>(std::min<int64_t>(PROJ_KMAX, o.k - j0));
             const int64_t tDst = blk[b].tOff + j0;
@@ -533,8 +534,8 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
             for (int64_t c = 0; c < nchunk; ++c) {
                 ProjItem it{};
                 it.blk = static_cast<int32_t>(b);
-                it.row0 = static_cast<int32_t>(c * PROJ_CHUNK);
-                it.rows = static_cast<int32_t>(std::min<int64_t>(PROJ_CHUNK, o.n - c * PROJ_CHUNK));
+                it.row0 = static_cast<int32_t>(c * chunk);
+                it.rows = static_cast<int32_t>(std::min<int64_t>(chunk, o.n - c * chunk));
                 it.j0 = static_cast<int32_t>(j0);
                 it.kc = kc;
                 it.out = nchunk > 1 ? tLen + pOff + c * kc : tDst;
@@ -650,7 +651,7 @@ int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void 
     DeviceGuard g(p->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t es = acc_size(p->storage);
-    if (p->storage != HM_F64 && nrhs >= 2 && d.bat.ready) {
+    if (p->storage != HM_F64 && nrhs >= 2 && d.bat.ready && !(p->flags & HM_NO_BATCHED)) {
         for (int64_t c0 = 0; c0 < nrhs; c0 += hm::NB) {
             const int nc = static_cast<int>(std::min<int64_t>(hm::NB, nrhs - c0));
             hipError_t e = hm::launch_batched(*p, d, nc, static_cast<const float *>(X) + c0 * ldx, ldx,
@@ -681,10 +682,45 @@ const char *hm_version(void) { return "hmat_amd 0.1 (gfx950)"; }
 
 int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *blocks,
                    const double *data, int64_t ndata, int storage, hm_plan_t *plan) {
-    using namespace hm;
+    return hm_plan_create_ex(nrows, ncols, nblocks, blocks, data, ndata, storage, nullptr, plan);
+}
+
+}  // extern "C"
+
+namespace {
+int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *blocks,
+                const double *data, int64_t ndata, int storage, const hm_config *cfg,
+                bool check_tiling, hm_plan_t *plan);
+}
+
+extern "C" int hm_plan_create_ex(int64_t nrows, int64_t ncols, int64_t nblocks,
+                                 const int64_t *blocks, const double *data, int64_t ndata,
+                                 int storage, const hm_config *cfg, hm_plan_t *plan) {
     g_err.clear();
+    return create_impl(nrows, ncols, nblocks, blocks, data, ndata, storage, cfg, true, plan);
+}
+
+namespace {
+// check_tiling: the leaves must tile the matrix exactly once (a plan made by
+// hm_plan_create_batch checks each member instead; the off-diagonal part of
+// its block-diagonal matrix is zero and has no leaves).
+int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *blocks,
+                const double *data, int64_t ndata, int storage, const hm_config *cfg,
+                bool check_tiling, hm_plan_t *plan) {
+    using namespace hm;
     if (!plan) return fail(HM_EINVAL, "plan output pointer is NULL");
     *plan = nullptr;
+    hm_config c{PROJ_CHUNK, WAVES, 0, 0};
+    if (cfg) {
+        c = *cfg;
+        if (c.proj_chunk == 0) c.proj_chunk = PROJ_CHUNK;
+        if (c.row_waves == 0) c.row_waves = WAVES;
+    }
+    if (c.proj_chunk < 64 || c.proj_chunk > 65536 || (c.proj_chunk & 63))
+        return fail(HM_EINVAL, "proj_chunk must be a multiple of 64 in [64, 65536]");
+    if (c.row_waves != 2 && c.row_waves != 4 && c.row_waves != 8)
+        return fail(HM_EINVAL, "row_waves must be 2, 4 or 8");
+    if (c.flags & ~(HM_NO_TRANSPOSE | HM_NO_BATCHED)) return fail(HM_EINVAL, "unknown flags");
     if (nrows < 0 || ncols < 0 || nblocks < 0 || ndata < 0)
         return fail(HM_EINVAL, "negative size");
     if (nrows >= (int64_t(1) << 31) || ncols >= (int64_t(1) << 31))
@@ -733,8 +769,8 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
         rowCover[o.row0 + o.m] -= o.n;
         fw[b] = o;
     }
-    if (area != nrows * ncols) return fail(HM_EINVAL, "leaves do not tile the matrix");
-    {
+    if (check_tiling && area != nrows * ncols) return fail(HM_EINVAL, "leaves do not tile the matrix");
+    if (check_tiling) {
         int64_t acc = 0;
         for (int64_t i = 0; i < nrows; ++i) {
             acc += rowCover[i];
@@ -751,6 +787,9 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
     if (!p) return fail(HM_ENOMEM, "host allocation failed");
     if (hipGetDevice(&p->device) != hipSuccess) return fail(HM_EHIP, "no HIP device");
     p->storage = storage;
+    p->proj_chunk = c.proj_chunk;
+    p->row_waves = c.row_waves;
+    p->flags = c.flags;
     p->nrows = nrows;
     p->ncols = ncols;
     p->ndense = ndense;
@@ -799,7 +838,7 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
             bw[b] = t;
         }
         build_direction(*p, p->fwd, nrows, ncols, fw, rkv.data(), data);
-        build_direction(*p, p->bwd, ncols, nrows, bw, rkv.data(), dT.data());
+        if (!(p->flags & HM_NO_TRANSPOSE)) build_direction(*p, p->bwd, ncols, nrows, bw, rkv.data(), dT.data());
         // adjoint gradient items over the caller's own arena offsets
         {
             constexpr int64_t GRAD_CHUNK = 16384;
@@ -827,9 +866,9 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
             p->lossbuf = dev_alloc(*p, sizeof(double));
             p->ndata = ndata;
         }
-        if (storage != HM_F64) {
+        if (storage != HM_F64 && !(p->flags & HM_NO_BATCHED)) {
             build_batched(*p, p->fwd, nrows, ncols, fw, rkv.data(), data);
-            build_batched(*p, p->bwd, ncols, nrows, bw, rkv.data(), dT.data());
+            if (!(p->flags & HM_NO_TRANSPOSE)) build_batched(*p, p->bwd, ncols, nrows, bw, rkv.data(), dT.data());
         }
     } catch (const HipFail &h) {
         const int code = (h.e == hipErrorOutOfMemory || h.e == hipErrorMemoryAllocation) ? HM_ENOMEM : HM_EHIP;
@@ -843,6 +882,9 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
     *plan = reinterpret_cast<hm_plan_t>(p.release());
     return HM_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int hm_plan_destroy(hm_plan_t plan) {
     free_plan(reinterpret_cast<hm::Plan *>(plan));
@@ -862,7 +904,10 @@ int hm_plan_info(hm_plan_t plan, int64_t *out, int nout) {
                                         p->storage,
                                         p->device_bytes,
                                         (p->dense_elems + p->rk_elems) * es,
-                                        p->maxrank};
+                                        p->maxrank,
+                                        p->proj_chunk,
+                                        p->row_waves,
+                                        p->flags};
     for (int i = 0; i < nout && i < HM_INFO_NFIELDS; ++i) out[i] = v[i];
     return HM_OK;
 }
@@ -885,6 +930,7 @@ int hm_matmat(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx, void *Y,
 int hm_matvec_t(hm_plan_t plan, const void *x, void *y, double alpha, double beta, void *stream) {
     hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
     if (!p) return fail(HM_EINVAL, "plan is NULL");
+    if (p->flags & HM_NO_TRANSPOSE) return fail(HM_EUNSUPPORTED, "plan was created with HM_NO_TRANSPOSE");
     if ((p->nrows > 0 && !x) || (p->ncols > 0 && !y)) return fail(HM_EINVAL, "NULL vector");
     return run_direction(p, p->bwd, 1, x, p->nrows > 0 ? p->nrows : 1, y,
                          p->ncols > 0 ? p->ncols : 1, alpha, beta, stream);
@@ -894,6 +940,7 @@ int hm_matmat_t(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx, void *
                 double alpha, double beta, void *stream) {
     hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
     if (!p) return fail(HM_EINVAL, "plan is NULL");
+    if (p->flags & HM_NO_TRANSPOSE) return fail(HM_EUNSUPPORTED, "plan was created with HM_NO_TRANSPOSE");
     return run_direction(p, p->bwd, nrhs, X, ldx, Y, ldy, alpha, beta, stream);
 }
 
@@ -901,6 +948,7 @@ int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx
                double *loss, void *stream) {
     hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
     if (!p) return fail(HM_EINVAL, "plan is NULL");
+    if (p->flags & HM_NO_TRANSPOSE) return fail(HM_EUNSUPPORTED, "plan was created with HM_NO_TRANSPOSE");
     if ((p->ncols > 0 && (!x || !dx)) || (p->nrows > 0 && (!b || !res)))
         return fail(HM_EINVAL, "NULL vector");
     if (res == b || res == x || dx == x || dx == res)
@@ -920,6 +968,161 @@ int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx
         *loss = h;
     }
     if (e != hipSuccess) return fail(HM_EHIP, std::string("hm_adjoint: ") + hipGetErrorString(e));
+    return HM_OK;
+}
+
+int hm_plan_create_batch(int64_t nmat, const int64_t *nrows, const int64_t *ncols,
+                         const int64_t *nblocks, const int64_t *const *blocks,
+                         const double *const *data, const int64_t *ndata, int storage,
+                         const hm_config *cfg, hm_plan_t *plan) {
+    g_err.clear();
+    if (!plan) return fail(HM_EINVAL, "plan output pointer is NULL");
+    *plan = nullptr;
+    if (nmat <= 0 || !nrows || !ncols || !nblocks || !blocks || !data || !ndata)
+        return fail(HM_EINVAL, "NULL argument or nmat <= 0");
+    int64_t R = 0, C = 0, NB_ = 0, ND = 0;
+    for (int64_t i = 0; i < nmat; ++i) {
+        if (nrows[i] < 0 || ncols[i] < 0 || nblocks[i] < 0 || ndata[i] < 0)
+            return fail(HM_EINVAL, "negative size in batch member");
+        if ((nblocks[i] > 0 && !blocks[i]) || (ndata[i] > 0 && !data[i]))
+            return fail(HM_EINVAL, "NULL array in batch member");
+        R += nrows[i];
+        C += ncols[i];
+        NB_ += nblocks[i];
+        ND += ndata[i];
+    }
+    // each member must be a valid plan input on its own (exact tiling, offsets
+    // inside its arena); the check runs on the host only
+    for (int64_t i = 0; i < nmat; ++i) {
+        int64_t area = 0;
+        std::vector<int64_t> cover(static_cast<size_t>(nrows[i]) + 1, 0);
+        for (int64_t b = 0; b < nblocks[i]; ++b) {
+            const int64_t *r = blocks[i] + 8 * b;
+            const bool bad = (r[0] != 0 && r[0] != 1) || r[1] < 0 || r[2] < 0 || r[3] < 0 || r[4] < 0 || r[5] < 0 ||
+                             r[6] < 0 || r[7] < 0 || r[1] + r[3] > nrows[i] || r[2] + r[4] > ncols[i] ||
+                             (r[0] == 0 ? r[6] + r[3] * r[4] > ndata[i]
+                                        : (r[6] + r[3] * r[5] > ndata[i] || r[7] + r[4] * r[5] > ndata[i]));
+            if (bad) {
+                char msg[120];
+                std::snprintf(msg, sizeof msg, "batch member %lld, block %lld: invalid leaf", (long long)i,
+                              (long long)b);
+                return fail(HM_EINVAL, msg);
+            }
+            area += r[3] * r[4];
+            cover[r[1]] += r[4];
+            cover[r[1] + r[3]] -= r[4];
+        }
+        int64_t acc = 0;
+        bool rows_ok = true;
+        for (int64_t k = 0; k < nrows[i]; ++k) {
+            acc += cover[k];
+            if (acc != ncols[i]) rows_ok = false;
+        }
+        if (area != nrows[i] * ncols[i] || !rows_ok) {
+            char msg[100];
+            std::snprintf(msg, sizeof msg, "batch member %lld: leaves do not tile the matrix", (long long)i);
+            return fail(HM_EINVAL, msg);
+        }
+    }
+    std::vector<int64_t> bt;
+    std::vector<double> dt;
+    try {
+        bt.resize(static_cast<size_t>(8 * NB_));
+        dt.resize(static_cast<size_t>(ND));
+    } catch (const std::bad_alloc &) {
+        return fail(HM_ENOMEM, "host allocation failed");
+    }
+    int64_t r0 = 0, c0 = 0, q = 0, d0 = 0;
+    for (int64_t i = 0; i < nmat; ++i) {
+        for (int64_t b = 0; b < nblocks[i]; ++b, ++q) {
+            const int64_t *src = blocks[i] + 8 * b;
+            int64_t *dst = bt.data() + 8 * q;
+            for (int f = 0; f < 8; ++f) dst[f] = src[f];
+            if (src[1] < 0 || src[2] < 0 || src[1] + src[3] > nrows[i] || src[2] + src[4] > ncols[i]) {
+                char msg[120];
+                std::snprintf(msg, sizeof msg, "batch member %lld, block %lld: outside its matrix",
+                              (long long)i, (long long)b);
+                return fail(HM_EINVAL, msg);
+            }
+            dst[1] += r0;
+            dst[2] += c0;
+            dst[6] += d0;
+            dst[7] += d0;
+        }
+        if (ndata[i] > 0) std::memcpy(dt.data() + d0, data[i], sizeof(double) * ndata[i]);
+        r0 += nrows[i];
+        c0 += ncols[i];
+        d0 += ndata[i];
+    }
+    return create_impl(R, C, NB_, bt.data(), dt.data(), ND, storage, cfg, false, plan);
+}
+
+struct hm_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+};
+
+int hm_graph_create(hm_plan_t plan, int op, int64_t nrhs, const void *X, int64_t ldx, void *Y,
+                    int64_t ldy, double alpha, double beta, hm_graph_t *graph) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    if (!graph) return fail(HM_EINVAL, "graph output pointer is NULL");
+    *graph = nullptr;
+    if (!p) return fail(HM_EINVAL, "plan is NULL");
+    if (op != HM_OP_N && op != HM_OP_T) return fail(HM_EINVAL, "op must be HM_OP_N or HM_OP_T");
+    if (op == HM_OP_T && (p->flags & HM_NO_TRANSPOSE))
+        return fail(HM_EUNSUPPORTED, "plan was created with HM_NO_TRANSPOSE");
+    DeviceGuard gd(p->device);
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+        return fail(HM_EHIP, "hipStreamCreate failed");
+    std::unique_ptr<hm_graph_s> g(new (std::nothrow) hm_graph_s());
+    if (!g) {
+        (void)hipStreamDestroy(s);
+        return fail(HM_ENOMEM, "host allocation failed");
+    }
+    g->device = p->device;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    int rc = HM_OK;
+    if (e == hipSuccess) {
+        rc = run_direction(p, op == HM_OP_N ? p->fwd : p->bwd, nrhs, X, ldx, Y, ldy, alpha, beta, s);
+        hipGraph_t gr = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s, &gr);
+        if (rc == HM_OK && e2 != hipSuccess) {
+            rc = fail(HM_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+        }
+        g->graph = gr;
+        if (rc == HM_OK) {
+            e = hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = fail(HM_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    } else {
+        rc = fail(HM_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    (void)hipStreamDestroy(s);
+    if (rc != HM_OK) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        return rc;
+    }
+    *graph = g.release();
+    return HM_OK;
+}
+
+int hm_graph_launch(hm_graph_t graph, void *stream) {
+    if (!graph) return fail(HM_EINVAL, "graph is NULL");
+    DeviceGuard gd(graph->device);
+    const hipError_t e = hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(HM_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    return HM_OK;
+}
+
+int hm_graph_destroy(hm_graph_t graph) {
+    if (!graph) return HM_OK;
+    DeviceGuard gd(graph->device);
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
     return HM_OK;
 }
 

@@ -39,7 +39,13 @@ def _stream_handle(stream) -> ctypes.c_void_p:
 class DeviceHmat:
     """One H-matrix resident on the GPU in a given storage precision."""
 
-    def __init__(self, flat: FlatHmat, dtype: str = "f64"):
+    def __init__(self, flat: FlatHmat, dtype: str = "f64", config: Optional[dict] = None,
+                 _batch=None):
+        """Upload ``flat`` in storage ``dtype``.
+
+        ``config`` (optional) sets ``hm_config`` fields: ``proj_chunk``,
+        ``row_waves``, ``no_transpose``, ``no_batched``.
+        """
         torch = _torch()
         if dtype not in _native.STORAGE:
             raise ValueError(f"dtype must be one of {sorted(_native.STORAGE)}")
@@ -47,17 +53,47 @@ class DeviceHmat:
         torch.cuda.init()
         self.flat = flat
         self.dtype = dtype
+        self.config = dict(config or {})
         self.vec_dtype = torch.float64 if dtype == "f64" else torch.float32
         self.device = torch.device("cuda", torch.cuda.current_device())
-        b = np.ascontiguousarray(flat.blocks, dtype=np.int64)
-        d = np.ascontiguousarray(flat.data, dtype=np.float64)
+        cfg = _native.HmConfig(int(self.config.get("proj_chunk", 0)), int(self.config.get("row_waves", 0)),
+                               (_native.HM_NO_TRANSPOSE if self.config.get("no_transpose") else 0)
+                               | (_native.HM_NO_BATCHED if self.config.get("no_batched") else 0), 0)
         h = ctypes.c_void_p()
-        rc = L.hm_plan_create(flat.nrows, flat.ncols, b.shape[0],
-                              b.ctypes.data_as(ctypes.c_void_p), d.ctypes.data_as(ctypes.c_void_p),
-                              d.size, _native.STORAGE[dtype], ctypes.byref(h))
-        _native.check(rc, "hm_plan_create")
+        if _batch is None:
+            b = np.ascontiguousarray(flat.blocks, dtype=np.int64)
+            d = np.ascontiguousarray(flat.data, dtype=np.float64)
+            rc = L.hm_plan_create_ex(flat.nrows, flat.ncols, b.shape[0],
+                                     b.ctypes.data_as(ctypes.c_void_p), d.ctypes.data_as(ctypes.c_void_p),
+                                     d.size, _native.STORAGE[dtype], ctypes.byref(cfg), ctypes.byref(h))
+            _native.check(rc, "hm_plan_create_ex")
+            self.nrows, self.ncols = flat.nrows, flat.ncols
+            self.ndata = int(flat.data.size)
+        else:
+            members = _batch
+            n = len(members)
+            arr = lambda vals: (ctypes.c_int64 * n)(*vals)
+            keep_b = [np.ascontiguousarray(m.blocks, dtype=np.int64) for m in members]
+            keep_d = [np.ascontiguousarray(m.data, dtype=np.float64) for m in members]
+            pb = (ctypes.c_void_p * n)(*[k.ctypes.data for k in keep_b])
+            pd = (ctypes.c_void_p * n)(*[k.ctypes.data for k in keep_d])
+            rc = L.hm_plan_create_batch(n, arr([m.nrows for m in members]), arr([m.ncols for m in members]),
+                                        arr([k.shape[0] for k in keep_b]), pb, pd, arr([k.size for k in keep_d]),
+                                        _native.STORAGE[dtype], ctypes.byref(cfg), ctypes.byref(h))
+            _native.check(rc, "hm_plan_create_batch")
+            self.nrows = sum(m.nrows for m in members)
+            self.ncols = sum(m.ncols for m in members)
+            self.ndata = int(sum(k.size for k in keep_d))
+            self.row_offsets = np.concatenate([[0], np.cumsum([m.nrows for m in members])])
+            self.col_offsets = np.concatenate([[0], np.cumsum([m.ncols for m in members])])
         self._h = h
-        self.nrows, self.ncols = flat.nrows, flat.ncols
+
+    @classmethod
+    def batch(cls, flats, dtype: str = "f32", config: Optional[dict] = None) -> "DeviceHmat":
+        """One plan for many independent H-matrices (hm_plan_create_batch):
+        one product launch covers all of them; member i acts on the slice
+        ``col_offsets[i]:col_offsets[i+1]`` of the concatenated vector."""
+        return cls(None, dtype, config, _batch=list(flats))
 
     @classmethod
     def from_hmat(cls, H, dtype: str = "f64") -> "DeviceHmat":
@@ -138,6 +174,24 @@ class DeviceHmat:
     def matmat_t(self, X, Y=None, alpha: float = 1.0, beta: float = 0.0, stream=None):
         return self.matmat(X, Y, alpha, beta, stream, transpose=True)
 
+    def graph(self, X, Y, alpha: float = 1.0, beta: float = 0.0, transpose: bool = False):
+        """Capture ``Y = beta*Y + alpha*op(H)*X`` on these buffers into a HIP
+        graph (hm_graph_create); returns a ``CapturedProduct``."""
+        m, n = (self.ncols, self.nrows) if transpose else (self.nrows, self.ncols)
+        if X.dim() == 1:
+            px, nrhs, ldx = self._vec(X, n, "X"), 1, max(1, n)
+            py, ldy = self._vec(Y, m, "Y"), max(1, m)
+        else:
+            px, nrhs, ldx = self._mat(X, n, "X")
+            py, nrhs2, ldy = self._mat(Y, m, "Y")
+            if nrhs2 != nrhs:
+                raise ValueError("X and Y have different column counts")
+        g = ctypes.c_void_p()
+        rc = _native.lib().hm_graph_create(self._h, _native.HM_OP_T if transpose else _native.HM_OP_N, nrhs,
+                                           px, ldx, py, ldy, float(alpha), float(beta), ctypes.byref(g))
+        _native.check(rc, "hm_graph_create")
+        return CapturedProduct(g, self, X, Y)
+
     def adjoint(self, x, b, grad: bool = True, loss: bool = True, stream=None):
         """Gradients of L = 0.5*||H x - b||^2 on the device.
 
@@ -150,7 +204,7 @@ class DeviceHmat:
         pb = self._vec(b, self.nrows, "b")
         res = torch.empty(self.nrows, dtype=self.vec_dtype, device=self.device)
         dx = torch.empty(self.ncols, dtype=self.vec_dtype, device=self.device)
-        g = torch.zeros(self.flat.data.size, dtype=self.vec_dtype, device=self.device) if grad else None
+        g = torch.zeros(self.ndata, dtype=self.vec_dtype, device=self.device) if grad else None
         L = ctypes.c_double(0.0)
         rc = _native.lib().hm_adjoint(self._h, px, pb, ctypes.c_void_p(res.data_ptr()),
                                       ctypes.c_void_p(dx.data_ptr()),
@@ -186,6 +240,28 @@ class DeviceHmat:
         X = self.new_cols(self.nrows, v.shape[0])
         X.copy_(torch.from_numpy(np.ascontiguousarray(v.T)).to(self.device, self.vec_dtype))
         return self.matmat_t(X).cpu().numpy().T.astype(np.float64)
+
+
+class CapturedProduct:
+    """A product captured into a HIP graph; ``launch()`` replays it."""
+
+    def __init__(self, handle, plan: DeviceHmat, X, Y):
+        self._g = handle
+        self._keep = (plan, X, Y)       # the graph refers to these buffers
+
+    def launch(self, stream=None) -> None:
+        _native.check(_native.lib().hm_graph_launch(self._g, _stream_handle(stream)), "hm_graph_launch")
+
+    def close(self) -> None:
+        if getattr(self, "_g", None) is not None and self._g.value:
+            _native.lib().hm_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class TransposedHmat:

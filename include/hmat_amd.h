@@ -64,8 +64,38 @@ enum hm_info_field {
     HM_INFO_DEVICE_BYTES,    /* device memory held by the plan                 */
     HM_INFO_FACTOR_BYTES,    /* bytes of leaves+factors one product streams   */
     HM_INFO_MAXRANK,
+    HM_INFO_PROJ_CHUNK,      /* hm_config.proj_chunk in effect                   */
+    HM_INFO_ROW_WAVES,       /* hm_config.row_waves in effect                    */
+    HM_INFO_FLAGS,           /* hm_config.flags                                  */
     HM_INFO_NFIELDS
 };
+
+/* hm_config.flags */
+enum hm_flags {
+    HM_NO_TRANSPOSE = 1,     /* do not build the H' structures (halves memory)   */
+    HM_NO_BATCHED = 2        /* do not build the multi-right-hand-side layout    */
+};
+
+/*
+ * Plan configuration (zero fields take the defaults).  These are the shapes
+ * the autotuner (hmatrix_amd.autotune) sweeps.
+ *   proj_chunk: rows of B one projection work item streams in the
+ *               single-right-hand-side product (multiple of 64, default 1024)
+ *   row_waves:  wavefronts per 64-row tile in the single-right-hand-side
+ *               row pass: 2, 4 (default) or 8
+ *   flags:      enum hm_flags
+ */
+typedef struct hm_config {
+    int32_t proj_chunk;
+    int32_t row_waves;
+    int32_t flags;
+    int32_t reserved;
+} hm_config;
+
+/* products a captured graph replays */
+enum hm_op { HM_OP_N = 0, HM_OP_T = 1 };
+
+typedef struct hm_graph_s *hm_graph_t;
 
 /*
  * Create a plan: validate the block table, pack the factors into the device
@@ -77,6 +107,23 @@ enum hm_info_field {
 int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks,
                    const int64_t *blocks, const double *data, int64_t ndata,
                    int storage, hm_plan_t *plan);
+
+/* hm_plan_create with an explicit configuration (cfg may be NULL). */
+int hm_plan_create_ex(int64_t nrows, int64_t ncols, int64_t nblocks,
+                      const int64_t *blocks, const double *data, int64_t ndata,
+                      int storage, const hm_config *cfg, hm_plan_t *plan);
+
+/*
+ * One plan for nmat independent H-matrices: member i's block table and arena
+ * (same format as hm_plan_create) are placed on the diagonal of a
+ * (sum nrows) x (sum ncols) matrix, so one product launch evaluates every
+ * member; member i's vectors start at the prefix sums of nrows / ncols.
+ * Replaces: a loop of H*y over separate Hmat objects (src/harithm.jl:325-329).
+ */
+int hm_plan_create_batch(int64_t nmat, const int64_t *nrows, const int64_t *ncols,
+                         const int64_t *nblocks, const int64_t *const *blocks,
+                         const double *const *data, const int64_t *ndata, int storage,
+                         const hm_config *cfg, hm_plan_t *plan);
 
 /* Release a plan and its device memory.  NULL is accepted. */
 int hm_plan_destroy(hm_plan_t plan);
@@ -133,6 +180,18 @@ int hm_matmat_t(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx,
  */
 int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx, void *grad,
                double *loss, void *stream);
+
+/*
+ * Capture one product (op = HM_OP_N: Y = beta*Y + alpha*H*X, HM_OP_T: with
+ * H') on fixed device buffers into a HIP graph; hm_graph_launch replays it on
+ * a stream with one launch.  The graph holds only kernel nodes.  The plan and
+ * the buffers must outlive the graph; replays on one plan must not overlap.
+ * No reference counterpart (launch-overhead form of hm_matmat).
+ */
+int hm_graph_create(hm_plan_t plan, int op, int64_t nrhs, const void *X, int64_t ldx,
+                    void *Y, int64_t ldy, double alpha, double beta, hm_graph_t *graph);
+int hm_graph_launch(hm_graph_t graph, void *stream);
+int hm_graph_destroy(hm_graph_t graph);
 
 /* Message of the last failing call on this host thread ("" if none). */
 const char *hm_last_error(void);

@@ -53,3 +53,30 @@ def test_plan_create_rejects(hm, blocks, data, shape, msg):
 def test_plan_create_rejects_bad_storage(hm):
     rc, err, h = _create([[0, 0, 0, 1, 1, 0, 0, 0]], [1.0], 1, 1, storage=9)
     assert rc == -1 and "storage" in err
+
+
+def test_plan_create_ex_rejects_bad_config(hm):
+    from hmatrix_amd import _native
+    L = _native.lib()
+    b = np.array([[0, 0, 0, 1, 1, 0, 0, 0]], dtype=np.int64)
+    d = np.ones(1)
+    for cfg in [_native.HmConfig(100, 4, 0, 0), _native.HmConfig(1024, 3, 0, 0), _native.HmConfig(1024, 4, 8, 0)]:
+        h = ctypes.c_void_p()
+        rc = L.hm_plan_create_ex(1, 1, 1, b.ctypes.data_as(ctypes.c_void_p), d.ctypes.data_as(ctypes.c_void_p),
+                                 1, 0, ctypes.byref(cfg), ctypes.byref(h))
+        assert rc == -1 and not h.value
+
+
+def test_batch_create_checks_each_member(hm):
+    from hmatrix_amd import _native
+    L = _native.lib()
+    good = np.array([[0, 0, 0, 2, 2, 0, 0, 0]], dtype=np.int64)
+    bad = np.array([[0, 0, 0, 2, 1, 0, 0, 0]], dtype=np.int64)     # covers half of its 2x2 matrix
+    d = np.ones(4)
+    for tables, msg in [((good, bad), "member 1"), ((bad, good), "member 0")]:
+        pb = (ctypes.c_void_p * 2)(*[t.ctypes.data for t in tables])
+        pd = (ctypes.c_void_p * 2)(d.ctypes.data, d.ctypes.data)
+        two = lambda v: (ctypes.c_int64 * 2)(v, v)
+        h = ctypes.c_void_p()
+        rc = L.hm_plan_create_batch(2, two(2), two(2), two(1), pb, pd, two(4), 0, None, ctypes.byref(h))
+        assert rc == -1 and msg in L.hm_last_error().decode() and not h.value

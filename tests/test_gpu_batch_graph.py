@@ -1,0 +1,108 @@
+"""Many H-matrices in one launch, HIP-graph replay, plan configurations and
+the autotuner, against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from tests.helpers import build, c1, c4_members, cauchy, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def members(hm):
+    return c4_members(hm, nmat=12, nlo=300, nhi=6000, seed=6)
+
+
+def test_batch_one_launch_matches_oracle(hm, orc, members):
+    import torch
+    d = hm.DeviceHmat.batch(members, "f32")
+    rng = np.random.default_rng(10)
+    xs = [rng.standard_normal(m.ncols) for m in members]
+    x = torch.from_numpy(np.concatenate(xs)).float().cuda()
+    y = d.matvec(x).cpu().numpy()
+    for i, m in enumerate(members):
+        yi = y[d.row_offsets[i]:d.row_offsets[i + 1]]
+        assert relerr(yi, orc.matvec(m, xs[i])) <= 2e-6, i
+
+
+def test_batch_f64_bitwise_close(hm, orc, members):
+    import torch
+    d = hm.DeviceHmat.batch(members[:5], "f64")
+    rng = np.random.default_rng(11)
+    xs = [rng.standard_normal(m.ncols) for m in members[:5]]
+    y = d.matvec(torch.from_numpy(np.concatenate(xs)).cuda()).cpu().numpy()
+    for i, m in enumerate(members[:5]):
+        assert relerr(y[d.row_offsets[i]:d.row_offsets[i + 1]], orc.matvec(m, xs[i])) <= 1e-13
+
+
+def test_graph_replay_matches_eager(hm, orc, members):
+    import torch
+    d = hm.DeviceHmat.batch(members, "f32")
+    x = torch.zeros(d.ncols, dtype=torch.float32, device="cuda")
+    y = torch.zeros(d.nrows, dtype=torch.float32, device="cuda")
+    g = d.graph(x, y)
+    for rep in range(4):
+        x.copy_(torch.randn(d.ncols, generator=torch.Generator().manual_seed(rep)).cuda())
+        ref = d.matvec(x).clone()            # eager product on the same input
+        torch.cuda.synchronize()
+        y.fill_(float("nan"))
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref), rep
+    g.close()
+
+
+def test_graph_batched_rhs_and_transpose(hm):
+    import torch
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    d = hm.DeviceHmat(f, "bf16x2")
+    X = d.new_cols(f.ncols, 32)
+    Y = d.new_cols(f.nrows, 32)
+    gN = d.graph(X, Y)
+    Xt = d.new_cols(f.nrows, 32)
+    Yt = d.new_cols(f.ncols, 32)
+    gT = d.graph(Xt, Yt, transpose=True)
+    for rep in range(3):
+        X.copy_(torch.randn(f.ncols, 32))
+        Xt.copy_(torch.randn(f.nrows, 32))
+        refN = d.matmat(X).clone()
+        refT = d.matmat_t(Xt).clone()
+        gN.launch()
+        gT.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(Y, refN) and torch.equal(Yt, refT), rep
+
+
+@pytest.mark.parametrize("cfg", [{"proj_chunk": 256, "row_waves": 2}, {"proj_chunk": 4096, "row_waves": 8},
+                                 {"proj_chunk": 64, "row_waves": 4}])
+def test_configs_keep_fp64_parity(hm, orc, cfg):
+    import torch
+    par, c, H = build(hm, np.sort(np.random.default_rng(1).uniform(0, 100, 20000)), cauchy, 1024, 64, 16)
+    f = hm.flatten(H)
+    d = hm.DeviceHmat(f, "f64", cfg)
+    assert d.info()["proj_chunk"] == cfg["proj_chunk"] and d.info()["row_waves"] == cfg["row_waves"]
+    x = np.random.default_rng(2).standard_normal(f.ncols)
+    assert relerr(d.matvec(torch.from_numpy(x).cuda()).cpu().numpy(), orc.matvec(f, x)) <= 1e-13
+    v = np.random.default_rng(3).standard_normal(f.nrows)
+    assert relerr(d.matvec(torch.from_numpy(v).cuda(), transpose=True).cpu().numpy(), orc.matvec_t(f, v)) <= 1e-13
+
+
+def test_no_transpose_flag(hm):
+    import torch
+    par, c, H = c1(hm, "aca2")
+    d = hm.DeviceHmat(hm.flatten(H), "f64", {"no_transpose": True})
+    from hmatrix_amd._native import HmError
+    with pytest.raises(HmError) as e:
+        d.matvec(torch.zeros(2000, dtype=torch.float64, device="cuda"), transpose=True)
+    assert e.value.code == -4
+
+
+def test_autotune_picks_a_config(hm, tmp_path):
+    from hmatrix_amd.autotune import autotune
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    best, t = autotune(f, "f64", chunks=(256, 1024), waves=(2, 4), reps=3, cache=str(tmp_path / "at.json"))
+    assert best["proj_chunk"] in (256, 1024) and best["row_waves"] in (2, 4) and len(t) == 4
+    best2, t2 = autotune(f, "f64", chunks=(256, 1024), waves=(2, 4), reps=3, cache=str(tmp_path / "at.json"))
+    assert best2 == best and t2 == t     # served from the cache
