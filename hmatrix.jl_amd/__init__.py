@@ -11,7 +11,9 @@ The directory is named ``hmatrix.jl_amd``; it is imported as
 ``hmatrix_amd`` (see ``__graft_entry__.package()``).
 """
 from .params import Hparams, Params
-from .cluster import Cluster, Hmat, NewHmat, FullMat, kernel_full, construct_cluster, bisect_cluster
+from .cluster import (Cluster, Hmat, NewHmat, FullMat, kernel_full, construct_cluster, bisect_cluster,
+                      UniformScaling, I, copy_hmat)
+from .solve import gmres, bicgstab, solve_ir
 from .lowrank import aca2, compress, rank_truncate, bbfmm
 from .flat import (FlatHmat, flatten, to_dense, to_fmat, Array, info, save_factors, load_factors,
                    KIND_DENSE, KIND_RK)
@@ -20,7 +22,8 @@ __all__ = [
     "Hparams", "Params", "Cluster", "Hmat", "NewHmat", "FullMat", "kernel_full",
     "construct_cluster", "bisect_cluster", "aca2", "compress", "rank_truncate", "bbfmm",
     "FlatHmat", "flatten", "to_dense", "to_fmat", "Array", "info", "save_factors",
-    "load_factors", "KIND_DENSE", "KIND_RK", "DeviceHmat", "hmatvec",
+    "load_factors", "KIND_DENSE", "KIND_RK", "DeviceHmat", "hmatvec", "UniformScaling", "I",
+    "copy_hmat", "gmres", "bicgstab", "solve_ir",
 ]
 
 

@@ -333,6 +333,19 @@ class Hmat:
 
     __rmatmul__ = __rmul__
 
+    def __add__(self, other):
+        """``H + u*I``: u added to the diagonal of the diagonal dense leaves
+        (src/harithm.jl:457-474); returns a new Hmat."""
+        if isinstance(other, UniformScaling):
+            if self.m != self.n:
+                raise ValueError("H + u*I needs a square H")
+            G = copy_hmat(self)
+            _add_diag(G, float(other.value))
+            return G
+        return NotImplemented
+
+    __radd__ = __add__
+
     @property
     def T(self):
         from .device import TransposedHmat
@@ -432,3 +445,45 @@ def NewHmat(params: Optional[Params] = None):
     adj = _Adjacency(par.Adj, c.P) if named else None
     H = __c1(c, par, MaxBlock, adj)
     return c, H
+
+
+class UniformScaling:
+    """``u*I`` for ``H + u*I`` (Julia's ``LinearAlgebra.I``)."""
+
+    __array_ufunc__ = None
+
+    def __init__(self, value: float = 1.0):
+        self.value = value
+
+    def __mul__(self, u):
+        return UniformScaling(self.value * u)
+
+    __rmul__ = __mul__
+
+
+I = UniformScaling(1.0)
+
+
+def copy_hmat(H: Hmat) -> Hmat:
+    """Deep copy of the leaves (utils.jl:129-159); cluster references shared."""
+    G = Hmat(m=H.m, n=H.n, s=H.s, t=H.t, is_rkmatrix=H.is_rkmatrix, is_fullmatrix=H.is_fullmatrix,
+             is_hmat=H.is_hmat, P=np.array(H.P, copy=True))
+    if H.is_fullmatrix:
+        G.C = np.array(H.C, copy=True)
+    elif H.is_rkmatrix:
+        G.A, G.B = np.array(H.A, copy=True, order="F"), np.array(H.B, copy=True, order="F")
+    else:
+        G.children = [[copy_hmat(c) for c in row] for row in H.children]
+    return G
+
+
+def _add_diag(H: Hmat, u: float) -> None:
+    if H.is_fullmatrix:
+        if H.m != H.n:
+            raise ValueError("a diagonal leaf is not square")
+        H.C[np.diag_indices(H.m)] += u
+    elif H.is_rkmatrix:
+        raise ValueError("H should not be a low-rank matrix")   # harithm.jl:462
+    else:
+        _add_diag(H.children[0][0], u)
+        _add_diag(H.children[1][1], u)

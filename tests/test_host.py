@@ -160,3 +160,10 @@ def test_product_fails_loudly_without_gpu(hm):
     par, c, H = build(hm, np.linspace(0, 1, 200), cauchy, 50, 16, 4)
     with pytest.raises(RuntimeError):
         H * np.ones(200)
+
+
+def test_add_identity(hm):
+    par, c, H = build(hm, np.linspace(0, 5, 500), cauchy, 125, 32, 6)
+    G = H + 10 * hm.I
+    assert np.allclose(hm.to_fmat(G), hm.to_fmat(H) + 10 * np.eye(500), rtol=0, atol=1e-14)
+    assert np.allclose(hm.to_fmat(H), hm.to_fmat(hm.copy_hmat(H)))      # H untouched
