@@ -99,6 +99,39 @@ def cpu_baseline(f, seconds: float):
     return n / dt, n, dt
 
 
+def profiled_traffic(storage: str):
+    """HBM bytes per step from the committed counter summary
+    (profiles/traffic_<storage>.json: FETCH_SIZE + WRITE_SIZE per step of the
+    same workload, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{storage}.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)["bytes_per_step"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def c4_report(hm, torch, stream) -> None:
+    """64 H-matrices, N log-uniform in [1000, 40000], 1/(1+((x-y)/l)^2),
+    l ~ U[0.2, 5], MaxRank in {4, 8, 16}; FP32; one launch (captured in a
+    HIP graph) for all members."""
+    from hmatrix_amd.workloads import uneven_batch
+    t0 = time.time()
+    members = uneven_batch(hm, nmat=64, nlo=1000, nhi=40000, seed=6)
+    log(f"C4 construction: {time.time() - t0:.1f}s, total N = {sum(m.nrows for m in members)}")
+    d = hm.DeviceHmat.batch(members, "f32", {"no_transpose": True, "no_batched": True})
+    x = torch.randn(d.ncols, dtype=torch.float32, device=d.device)
+    y = torch.zeros(d.nrows, dtype=torch.float32, device=d.device)
+    g = d.graph(x, y)
+    ms = timed(torch, lambda i: g.launch(stream), 50, 5, stream)
+    ms_eager = timed(torch, lambda i: d.matvec(x, y, stream=stream), 50, 5, stream)
+    elems = sum(m.stored_elements() for m in members)
+    log(f"C4 batch (64 matrices, FP32, 1 rhs each): graph {ms * 1e3:.1f} us, eager {ms_eager * 1e3:.1f} us "
+        f"per launch; {64 / (ms * 1e-3):.0f} H-matvec/s; {(4 * elems + 8 * d.nrows) / (ms * 1e-3) / 1e9:.0f} GB/s")
+    g.close()
+    d.close()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +144,11 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--sample-rows", type=int, default=4096)
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="after the measurement, run 3 device copies of this many MB (the plain copy "
+                         "kernel the FETCH_SIZE/WRITE_SIZE counters are scaled against)")
+    ap.add_argument("--c4", action="store_true",
+                    help="also time the uneven 64-matrix batch (one launch) and report it on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the product is "
@@ -167,6 +205,25 @@ def main() -> int:
     log(f"plain f64: {ms64 * 1e3:.1f} us per {nrhs} rhs; relerr vs dense {relerr_dense64:.3e}; "
         f"batched vs plain {relerr_h64:.3e}")
 
+    # drop-in path with host buffers (PCIe-inclusive; never the reported value)
+    t0 = time.perf_counter()
+    plans[0].mul(RHS)
+    torch.cuda.synchronize()
+    pcie_s = time.perf_counter() - t0
+    log(f"host-buffer H*X (upload + product + download): {pcie_s * 1e3:.1f} ms")
+
+    if args.c4:
+        c4_report(hm, torch, stream)
+
+    if args.calibrate_copy_mb > 0:
+        n = args.calibrate_copy_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b_ = torch.empty_like(a)
+        for _ in range(3):
+            b_.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration: 3 copies of {args.calibrate_copy_mb} MB done")
+
     bytes_b = algorithmic_bytes(f, args.storage, nrhs, 4)
     bytes_64 = algorithmic_bytes(f, "f64", 1, 8)
     achieved = bytes_b / (ms * 1e-3) / 1e9
@@ -197,7 +254,7 @@ def main() -> int:
                    "N": N, "nrhs": nrhs, "leaves": f.nblocks, "dense_leaves": f.counts()[0],
                    "rk_leaves": f.counts()[1], "replicas": R},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.storage)},
         "cpu_baseline": cpu,
         "relerr_vs_dense_fp64": relerr_dense,
         "relerr_vs_fp64_hmatrix": relerr_h64,
@@ -205,6 +262,7 @@ def main() -> int:
                        "GB/s": bytes_64 / (ms64 * 1e-3 / nrhs) / 1e9,
                        "relerr_vs_dense_fp64": relerr_dense64},
         "speedup_vs_plain_fp64": value / plain_mv_s,
+        "host_buffer_product_ms": pcie_s * 1e3,
         "algorithmic_bytes_per_step": bytes_b,
     }
     print(json.dumps(line), flush=True)

@@ -47,18 +47,8 @@ def relerr(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+
 def c4_members(hm, nmat=64, nlo=1000, nhi=40000, seed=6):
-    """Uneven batch: N log-uniform in [nlo, nhi], kernel 1/(1+((x-y)/l)^2) with
-    l ~ U[0.2, 5], MaxRank in {4, 8, 16}, sorted uniform points on [0, 100],
-    MinBlock 64, MaxBlock N/4, weak admissibility."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(nmat):
-        N = int(round(np.exp(rng.uniform(np.log(nlo), np.log(nhi)))))
-        l = float(rng.uniform(0.2, 5.0))
-        mr = int(rng.choice([4, 8, 16]))
-        X = np.sort(rng.uniform(0.0, 100.0, N))
-        kern = (lambda L: (lambda x, y: 1.0 / (1.0 + ((x - y) / L) ** 2)))(l)
-        par, c, H = build(hm, X, kern, -1, 64, mr)
-        out.append(hm.flatten(H))
-    return out
+    """Uneven batch of independent H-matrices (see hmatrix_amd.workloads)."""
+    from hmatrix_amd.workloads import uneven_batch
+    return uneven_batch(hm, nmat=nmat, nlo=nlo, nhi=nhi, seed=seed)
