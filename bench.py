@@ -61,6 +61,26 @@ def algorithmic_bytes(f, storage: str, nrhs: int, vec_bytes: int) -> int:
     return fb * (de + re) + 2 * vec_bytes * f.nrows * nrhs
 
 
+def kernel_bytes(f, storage: str, nrhs: int, vec_bytes: int) -> dict:
+    """Algorithmic bytes of one launch of each kernel of a product:
+    projection  = b_f * sum_rk n k + b_x N nrhs + b_t T nrhs   (reads B, X; writes t)
+    row pass    = b_f * (sum_dense m n + sum_rk m k) + (b_x + b_y) N nrhs + b_t T nrhs
+                  (reads dense leaves, A, X, t; writes Y)
+    reduction   = 0 (folds the projection's partial sums; no algorithmic bytes)
+    with T = sum_rk k the length of t."""
+    b = f.blocks
+    dense = b[:, 0] == 0
+    de = int((b[dense, 3] * b[dense, 4]).sum())
+    ae = int((b[~dense, 3] * b[~dense, 5]).sum())
+    be = int((b[~dense, 4] * b[~dense, 5]).sum())
+    T = int(b[~dense, 5].sum())
+    fb = {"f64": 8, "f32": 4, "bf16": 2, "bf16x2": 4}[storage]
+    nv = vec_bytes * nrhs
+    return {"proj": fb * be + nv * (f.ncols + T),
+            "reduce": 0,
+            "rows": fb * (de + ae) + nv * (f.ncols + T + f.nrows)}
+
+
 def dense_rows_reference(par, c, RHS, rows):
     """FP64 dense kernel rows: (K(x_rows, x) @ RHS) in cluster order."""
     Xc = np.asarray(c.X)
@@ -72,10 +92,12 @@ def dense_rows_reference(par, c, RHS, rows):
     return out
 
 
-def timed(torch, fn, steps: int, warmup: int, stream):
+def timed(torch, fn, steps: int, warmup: int, stream, after_warmup=None):
     for i in range(warmup):
         fn(i)
     torch.cuda.synchronize()
+    if after_warmup is not None:
+        after_warmup()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -99,14 +121,14 @@ def cpu_baseline(f, seconds: float):
     return n / dt, n, dt
 
 
-def profiled_traffic(storage: str):
-    """HBM bytes per step from the committed counter summary
-    (profiles/traffic_<storage>.json: FETCH_SIZE + WRITE_SIZE per step of the
-    same workload, scaled by the copy-kernel calibration), or None."""
+def profiled_traffic(storage: str, kernel: str):
+    """HBM bytes per launch of one kernel from the committed counter summary
+    (profiles/traffic_<storage>.json: FETCH_SIZE and WRITE_SIZE per dispatch
+    on this workload, scaled by the copy-kernel calibration), or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{storage}.json")
     try:
         with open(path) as fh:
-            return json.load(fh)["bytes_per_step"]
+            return json.load(fh)["bytes_per_launch"][kernel]
     except (OSError, ValueError, KeyError):
         return None
 
@@ -178,8 +200,28 @@ def main() -> int:
     def step_batched(i):
         plans[i % R].matmat(Xs[i % R], Ys[i % R], stream=stream)
 
+    def start_kernel_timing():
+        for p in plans:
+            p.timing(True)
+            p.timing_read()
+
     ms = timed(torch, step_batched, args.steps, args.warmup, stream)
-    log(f"batched {args.storage}: {ms * 1e3:.1f} us per product of {nrhs} rhs")
+    # kernel durations: the same steps again, with HIP events recorded by the
+    # library on `stream` around each of the three kernels (the events add a
+    # few microseconds of gaps per step, so the step time above is taken
+    # without them)
+    ms_ev = timed(torch, step_batched, args.steps, 0, stream, start_kernel_timing)
+    ph = {"proj_ms": 0.0, "reduce_ms": 0.0, "rows_ms": 0.0, "launches": 0}
+    for p in plans:
+        r = p.timing_read()
+        p.timing(False)
+        for k in ph:
+            ph[k] += r[k]
+    nl = max(ph["launches"], 1)
+    kus = {k: ph[f"{k}_ms"] * 1e3 / nl for k in ("proj", "reduce", "rows")}
+    log(f"batched {args.storage}: {ms * 1e3:.1f} us per product of {nrhs} rhs ({ms_ev * 1e3:.1f} with "
+        f"kernel events); kernels (us/launch, "
+        f"{ph['launches']} launches): proj {kus['proj']:.1f} reduce {kus['reduce']:.1f} rows {kus['rows']:.1f}")
 
     # accuracy of the batched result
     Yb = Ys[0].cpu().numpy().astype(np.float64)
@@ -226,7 +268,10 @@ def main() -> int:
 
     bytes_b = algorithmic_bytes(f, args.storage, nrhs, 4)
     bytes_64 = algorithmic_bytes(f, "f64", 1, 8)
-    achieved = bytes_b / (ms * 1e-3) / 1e9
+    kb = kernel_bytes(f, args.storage, nrhs, 4)
+    kgbs = {k: (kb[k] / (kus[k] * 1e-6) / 1e9 if kus[k] > 0 else None) for k in kb}
+    achieved = kgbs["rows"]
+    step_gbs = bytes_b / (ms * 1e-3) / 1e9
     value = nrhs / (ms * 1e-3)
     plain_mv_s = nrhs / (ms64 * 1e-3)
     cpu = None
@@ -254,7 +299,19 @@ def main() -> int:
                    "N": N, "nrhs": nrhs, "leaves": f.nblocks, "dense_leaves": f.counts()[0],
                    "rk_leaves": f.counts()[1], "replicas": R},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.storage)},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(args.storage, "kb_rows"),
+                     "kernel": "kb_rows (batched row pass)", "algorithmic_bytes": kb["rows"],
+                     "us_per_launch": kus["rows"]},
+        "kernels": {"kb_proj": {"us": kus["proj"], "algorithmic_bytes": kb["proj"], "GB/s": kgbs["proj"],
+                                "traffic": profiled_traffic(args.storage, "kb_proj")},
+                    "kb_reduce": {"us": kus["reduce"], "algorithmic_bytes": 0,
+                                  "traffic": profiled_traffic(args.storage, "kb_reduce")},
+                    "kb_rows": {"us": kus["rows"], "algorithmic_bytes": kb["rows"], "GB/s": kgbs["rows"],
+                                "traffic": profiled_traffic(args.storage, "kb_rows")}},
+        "ms_per_step_with_kernel_events": ms_ev,
+        "step_GBs": step_gbs,
+        "step_frac": step_gbs / HBM_PEAK_GBS,
         "cpu_baseline": cpu,
         "relerr_vs_dense_fp64": relerr_dense,
         "relerr_vs_fp64_hmatrix": relerr_h64,

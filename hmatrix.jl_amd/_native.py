@@ -47,6 +47,8 @@ SIGNATURES = {
                                        ctypes.POINTER(_P)]),
     "hm_graph_launch": (ctypes.c_int, [_P, _P]),
     "hm_graph_destroy": (ctypes.c_int, [_P]),
+    "hm_plan_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "hm_plan_timing_read": (ctypes.c_int, [_P, ctypes.POINTER(_D), ctypes.c_int, ctypes.POINTER(_I64)]),
     "hm_last_error": (ctypes.c_char_p, []),
     "hm_version": (ctypes.c_char_p, []),
 }

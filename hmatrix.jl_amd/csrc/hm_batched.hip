@@ -505,38 +505,45 @@ __global__ __launch_bounds__(256) void kb_rows(int64_t ntiles, int64_t wg_per_xc
 
 template <int MODE>
 static hipError_t run_b(const Direction &d, const void *zeros, int ncol, const float *X,
-                        int64_t ldx, float *Y, int64_t ldy, float alpha, float beta, hipStream_t s) {
+                        int64_t ldx, float *Y, int64_t ldy, float alpha, float beta, hipStream_t s,
+                        hipEvent_t *ev) {
     const Batched &b = d.bat;
+    hipError_t e = hipSuccess;
     if (b.nptiles > 0) {
         hipLaunchKernelGGL(kb_proj<MODE>, dim3(static_cast<unsigned>(b.nptiles)), dim3(256), 0, s,
                            b.ptiles, b.pitems, b.vslc, static_cast<const float *>(zeros), X, ldx,
                            d.ncols, ncol, b.t);
-        hipError_t e = hipGetLastError();
+        e = hipGetLastError();
         if (e != hipSuccess) return e;
-        if (b.nred > 0) {
-            hipLaunchKernelGGL(kb_reduce, dim3(static_cast<unsigned>(b.nred)), dim3(256), 0, s, b.red,
-                               static_cast<const float *>(b.t) + b.tLen, b.t);
-            e = hipGetLastError();
-            if (e != hipSuccess) return e;
-        }
     }
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if (b.nptiles > 0 && b.nred > 0) {
+        hipLaunchKernelGGL(kb_reduce, dim3(static_cast<unsigned>(b.nred)), dim3(256), 0, s, b.red,
+                           static_cast<const float *>(b.t) + b.tLen, b.t);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (ev) (void)hipEventRecord(ev[1], s);
     if (b.ntiles > 0) {
         const int64_t per = (b.ntiles + 7) / 8;
         hipLaunchKernelGGL(kb_rows<MODE>, dim3(static_cast<unsigned>(8 * per)), dim3(256), 0, s,
                            b.ntiles, per, b.blobs, b.blobx, b.frag, static_cast<const float *>(zeros),
                            static_cast<const float *>(b.t),
                            X, ldx, Y, ldy, ncol, alpha, beta, beta == 0.0f ? 1 : 0);
-        return hipGetLastError();
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
+    if (ev) (void)hipEventRecord(ev[2], s);
     return hipSuccess;
 }
 
 hipError_t launch_batched(const Plan &p, const Direction &d, int ncol, const float *X, int64_t ldx,
-                          float *Y, int64_t ldy, float alpha, float beta, hipStream_t s) {
+                          float *Y, int64_t ldy, float alpha, float beta, hipStream_t s,
+                          hipEvent_t *ev) {
     switch (p.storage) {
-        case MODE_F32: return run_b<MODE_F32>(d, p.zeros, ncol, X, ldx, Y, ldy, alpha, beta, s);
-        case MODE_BF16: return run_b<MODE_BF16>(d, p.zeros, ncol, X, ldx, Y, ldy, alpha, beta, s);
-        case MODE_BF16X2: return run_b<MODE_BF16X2>(d, p.zeros, ncol, X, ldx, Y, ldy, alpha, beta, s);
+        case MODE_F32: return run_b<MODE_F32>(d, p.zeros, ncol, X, ldx, Y, ldy, alpha, beta, s, ev);
+        case MODE_BF16: return run_b<MODE_BF16>(d, p.zeros, ncol, X, ldx, Y, ldy, alpha, beta, s, ev);
+        case MODE_BF16X2: return run_b<MODE_BF16X2>(d, p.zeros, ncol, X, ldx, Y, ldy, alpha, beta, s, ev);
         default: return hipErrorInvalidValue;
     }
 }

@@ -209,6 +209,19 @@ struct GradItem {
     int32_t kind;
 };
 
+// Per-phase kernel timing (instrumentation): each product records four
+// events on its stream -- before the projection kernel, after it, after the
+// reduction and after the row pass -- into a ring of TIMING_RING records;
+// reading folds the elapsed times into ms[] and empties the ring.
+constexpr int TIMING_RING = 1024;
+struct Timing {
+    bool on = false;
+    std::vector<hipEvent_t> ev;   // TIMING_RING * 4 events, created on enable
+    int64_t used = 0;             // records in the ring
+    double ms[3] = {0.0, 0.0, 0.0};
+    int64_t count = 0;            // records folded into ms
+};
+
 struct Plan {
     int device = 0;
     int storage = 0;
@@ -231,17 +244,21 @@ struct Plan {
     void *lossbuf = nullptr;       // one double
     int64_t ndata = 0;
     std::vector<void *> allocs;
+    Timing timing;
 };
 
 // Launchers (hm_kernels.hip).  Vectors are double for HM_F64, float otherwise.
+// ev (may be NULL): three events recorded after the projection, the
+// reduction and the row pass.
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y,
-                         double alpha, double beta, hipStream_t s);
+                         double alpha, double beta, hipStream_t s, hipEvent_t *ev = nullptr);
 // Adjoint: gradients from res, x and the t vectors the two products left in
 // fwd.t (t = B' x) and bwd.t (s = A' res); optional loss into p.lossbuf.
 hipError_t launch_adjoint_grads(const Plan &p, const void *x, const void *res, void *grad,
                                 bool want_loss, hipStream_t s);
 // Batched: one panel of ncol <= NB right-hand sides, float vectors.
 hipError_t launch_batched(const Plan &p, const Direction &d, int ncol, const float *X, int64_t ldx,
-                          float *Y, int64_t ldy, float alpha, float beta, hipStream_t s);
+                          float *Y, int64_t ldy, float alpha, float beta, hipStream_t s,
+                          hipEvent_t *ev = nullptr);
 
 }  // namespace hm

@@ -277,11 +277,12 @@ hipError_t launch_adjoint_grads(const Plan &p, const void *x, const void *res, v
 
 template <class St>
 static hipError_t run(const Plan &p, const Direction &d, const void *xv, void *yv, double alpha,
-                      double beta, hipStream_t s) {
+                      double beta, hipStream_t s, hipEvent_t *ev) {
     using A = typename St::acc;
     const A *x = static_cast<const A *>(xv);
     A *y = static_cast<A *>(yv);
     A *t = static_cast<A *>(d.t);
+    hipError_t e = hipSuccess;
     if (d.nproj > 0) {
         // The scratch buffer is [t (tLen) | partials (partLen)]: items that
         // cover a whole leaf write t directly, chunked items write partials
@@ -289,17 +290,19 @@ static hipError_t run(const Plan &p, const Direction &d, const void *xv, void *y
         const unsigned g = static_cast<unsigned>((d.nproj + WAVES - 1) / WAVES);
         hipLaunchKernelGGL(k_proj<St>, dim3(g), dim3(64 * WAVES), 0, s, d.blk, d.proj, d.nproj,
                            p.fac, p.fac_lo, x, t);
-        hipError_t e = hipGetLastError();
+        e = hipGetLastError();
         if (e != hipSuccess) return e;
-        if (d.nred > 0) {
-            const int64_t thr = d.nred * PROJ_KMAX;
-            const unsigned g2 = static_cast<unsigned>((thr + 255) / 256);
-            hipLaunchKernelGGL(k_reduce<A>, dim3(g2), dim3(256), 0, s, d.red, d.nred,
-                               static_cast<const A *>(t) + d.tLen, t);
-            e = hipGetLastError();
-            if (e != hipSuccess) return e;
-        }
     }
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if (d.nproj > 0 && d.nred > 0) {
+        const int64_t thr = d.nred * PROJ_KMAX;
+        const unsigned g2 = static_cast<unsigned>((thr + 255) / 256);
+        hipLaunchKernelGGL(k_reduce<A>, dim3(g2), dim3(256), 0, s, d.red, d.nred,
+                           static_cast<const A *>(t) + d.tLen, t);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (ev) (void)hipEventRecord(ev[1], s);
     if (d.ntiles > 0) {
         const dim3 grid(static_cast<unsigned>(d.ntiles));
         const A *tt = static_cast<const A *>(t);
@@ -314,18 +317,20 @@ static hipError_t run(const Plan &p, const Direction &d, const void *xv, void *y
         else
             hipLaunchKernelGGL((k_rows<St, 4>), grid, dim3(256), 0, s, d.tiles, d.mult, d.stream, d.stream_lo,
                                x, tt, y, al, be, bz);
-        return hipGetLastError();
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
+    if (ev) (void)hipEventRecord(ev[2], s);
     return hipSuccess;
 }
 
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y, double alpha,
-                         double beta, hipStream_t s) {
+                         double beta, hipStream_t s, hipEvent_t *ev) {
     switch (p.storage) {
-        case 0: return run<StF64>(p, d, x, y, alpha, beta, s);
-        case 1: return run<StF32>(p, d, x, y, alpha, beta, s);
-        case 2: return run<StBF16>(p, d, x, y, alpha, beta, s);
-        case 3: return run<StBF16x2>(p, d, x, y, alpha, beta, s);
+        case 0: return run<StF64>(p, d, x, y, alpha, beta, s, ev);
+        case 1: return run<StF32>(p, d, x, y, alpha, beta, s, ev);
+        case 2: return run<StBF16>(p, d, x, y, alpha, beta, s, ev);
+        case 3: return run<StBF16x2>(p, d, x, y, alpha, beta, s, ev);
         default: return hipErrorInvalidValue;
     }
 }

@@ -621,8 +621,30 @@ struct DeviceGuard {
 void free_plan(hm::Plan *p) {
     if (!p) return;
     DeviceGuard g(p->device);
+    for (hipEvent_t e : p->timing.ev) (void)hipEventDestroy(e);
     for (void *a : p->allocs) (void)hipFree(a);
     delete p;
+}
+
+// Wait for every recorded event set and add its three phase durations to
+// the plan's sums; empties the ring.  The caller holds the plan's device.
+int fold_timing(hm::Plan *p) {
+    hm::Timing &t = p->timing;
+    for (int64_t r = 0; r < t.used; ++r) {
+        hipEvent_t *ev = t.ev.data() + 4 * r;
+        const hipError_t e = hipEventSynchronize(ev[3]);
+        if (e != hipSuccess) {
+            t.used = 0;
+            return fail(HM_EHIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+        }
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) == hipSuccess) t.ms[k] += ms;
+        }
+        ++t.count;
+    }
+    t.used = 0;
+    return HM_OK;
 }
 
 int check_vectors(const hm::Plan *p, const void *x, const void *y) {
@@ -651,12 +673,27 @@ int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void 
     DeviceGuard g(p->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t es = acc_size(p->storage);
+    // Phase timing records one set of four events per launch sequence; none
+    // while the stream is being captured into a graph.
+    bool timed = p->timing.on;
+    if (timed) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) timed = false;
+    }
+    auto record = [&]() -> hipEvent_t * {
+        if (!timed) return nullptr;
+        if (p->timing.used == hm::TIMING_RING && fold_timing(p) != HM_OK) return nullptr;
+        hipEvent_t *ev = p->timing.ev.data() + 4 * p->timing.used++;
+        (void)hipEventRecord(ev[0], s);
+        return ev + 1;
+    };
     if (p->storage != HM_F64 && nrhs >= 2 && d.bat.ready && !(p->flags & HM_NO_BATCHED)) {
         for (int64_t c0 = 0; c0 < nrhs; c0 += hm::NB) {
             const int nc = static_cast<int>(std::min<int64_t>(hm::NB, nrhs - c0));
             hipError_t e = hm::launch_batched(*p, d, nc, static_cast<const float *>(X) + c0 * ldx, ldx,
                                               static_cast<float *>(Y) + c0 * ldy, ldy,
-                                              static_cast<float>(alpha), static_cast<float>(beta), s);
+                                              static_cast<float>(alpha), static_cast<float>(beta), s,
+                                              record());
             if (e != hipSuccess)
                 return fail(HM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
         }
@@ -665,7 +702,7 @@ int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void 
     for (int64_t c = 0; c < nrhs; ++c) {
         const void *xc = static_cast<const char *>(X) + c * ldx * es;
         void *yc = static_cast<char *>(Y) + c * ldy * es;
-        hipError_t e = hm::launch_matvec(*p, d, xc, yc, alpha, beta, s);
+        hipError_t e = hm::launch_matvec(*p, d, xc, yc, alpha, beta, s, record());
         if (e != hipSuccess)
             return fail(HM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
     }
@@ -909,6 +946,44 @@ int hm_plan_info(hm_plan_t plan, int64_t *out, int nout) {
                                         p->row_waves,
                                         p->flags};
     for (int i = 0; i < nout && i < HM_INFO_NFIELDS; ++i) out[i] = v[i];
+    return HM_OK;
+}
+
+int hm_plan_timing(hm_plan_t plan, int enable) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    if (!p) return fail(HM_EINVAL, "plan is NULL");
+    DeviceGuard g(p->device);
+    hm::Timing &t = p->timing;
+    if (enable && t.ev.empty()) {
+        t.ev.resize(4 * static_cast<size_t>(hm::TIMING_RING), nullptr);
+        for (hipEvent_t &e : t.ev) {
+            if (hipEventCreate(&e) != hipSuccess) {
+                for (hipEvent_t &f : t.ev)
+                    if (f) (void)hipEventDestroy(f);
+                t.ev.clear();
+                return fail(HM_EHIP, "hipEventCreate failed");
+            }
+        }
+    }
+    if (!enable && t.used > 0) {
+        const int rc = fold_timing(p);
+        if (rc) return rc;
+    }
+    t.on = enable != 0;
+    return HM_OK;
+}
+
+int hm_plan_timing_read(hm_plan_t plan, double *ms, int nms, int64_t *count) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    if (!p || (!ms && nms > 0)) return fail(HM_EINVAL, "NULL argument");
+    DeviceGuard g(p->device);
+    const int rc = fold_timing(p);
+    if (rc) return rc;
+    hm::Timing &t = p->timing;
+    for (int k = 0; k < nms && k < 3; ++k) ms[k] = t.ms[k];
+    if (count) *count = t.count;
+    t.ms[0] = t.ms[1] = t.ms[2] = 0.0;
+    t.count = 0;
     return HM_OK;
 }
 

@@ -115,6 +115,20 @@ class DeviceHmat:
         _native.check(_native.lib().hm_plan_info(self._h, out, len(out)), "hm_plan_info")
         return dict(zip(_native.INFO_FIELDS, list(out)))
 
+    def timing(self, enable: bool = True) -> None:
+        """Record HIP events around the projection, reduction and row-pass
+        kernels of every later product on this plan (not graph replays)."""
+        _native.check(_native.lib().hm_plan_timing(self._h, 1 if enable else 0), "hm_plan_timing")
+
+    def timing_read(self) -> dict:
+        """Summed kernel milliseconds since the last read, per phase, and the
+        number of launch sequences they cover; waits for the recorded events."""
+        ms = (ctypes.c_double * 3)()
+        n = ctypes.c_int64(0)
+        _native.check(_native.lib().hm_plan_timing_read(self._h, ms, 3, ctypes.byref(n)),
+                      "hm_plan_timing_read")
+        return {"proj_ms": ms[0], "reduce_ms": ms[1], "rows_ms": ms[2], "launches": n.value}
+
     # ------------------------------------------------------------------
     def _vec(self, t, n: int, name: str):
         torch = _torch()

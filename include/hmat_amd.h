@@ -193,6 +193,20 @@ int hm_graph_create(hm_plan_t plan, int op, int64_t nrhs, const void *X, int64_t
 int hm_graph_launch(hm_graph_t graph, void *stream);
 int hm_graph_destroy(hm_graph_t graph);
 
+/*
+ * Per-kernel timing (instrumentation; no reference counterpart).  With
+ * enable != 0 every later product on this plan (hm_matvec, hm_matmat and the
+ * transposed forms; not graph replays, not hm_adjoint) records HIP events on
+ * its stream around its three kernels: projection, reduction, row pass.
+ * hm_plan_timing_read waits for the recorded events, writes the summed
+ * milliseconds of those phases into ms[0..min(nms,3)-1] and the number of
+ * launch sequences (one per panel of 32 right-hand sides, or per column on
+ * the single-right-hand-side path) into *count (may be NULL), and clears the
+ * sums.  Disabling folds pending records first.
+ */
+int hm_plan_timing(hm_plan_t plan, int enable);
+int hm_plan_timing_read(hm_plan_t plan, double *ms, int nms, int64_t *count);
+
 /* Message of the last failing call on this host thread ("" if none). */
 const char *hm_last_error(void);
 

@@ -80,3 +80,11 @@ def test_batch_create_checks_each_member(hm):
         h = ctypes.c_void_p()
         rc = L.hm_plan_create_batch(2, two(2), two(2), two(1), pb, pd, two(4), 0, None, ctypes.byref(h))
         assert rc == -1 and msg in L.hm_last_error().decode() and not h.value
+
+
+def test_timing_rejects_null_plan(hm):
+    from hmatrix_amd import _native
+    L = _native.lib()
+    ms = (ctypes.c_double * 3)()
+    assert L.hm_plan_timing(None, 1) == -1
+    assert L.hm_plan_timing_read(None, ms, 3, None) == -1

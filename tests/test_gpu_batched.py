@@ -80,3 +80,23 @@ def test_ragged_small(hm, orc):
     f = hm.flatten(H)
     R = np.random.default_rng(5).standard_normal((f.ncols, 3))
     assert relerr(_run(hm, f, "bf16x2", R), orc.matmat(f, R)) <= 2e-5
+
+
+def test_kernel_timing_counts_launches(hm, orc, c1f):
+    """hm_plan_timing records one event set per 32-column panel; the phase
+    sums are positive and the product is unchanged."""
+    import torch
+    d = hm.DeviceHmat(c1f, "bf16x2")
+    X = np.random.default_rng(12).standard_normal((c1f.ncols, 40))
+    Xd = d.new_cols(c1f.ncols, 40)
+    Xd.copy_(torch.from_numpy(X).float())
+    d.timing(True)
+    Y = d.matmat(Xd)
+    Y = d.matmat(Xd)
+    r = d.timing_read()
+    d.timing(False)
+    assert r["launches"] == 4                      # 2 products x 2 panels (32 + 8)
+    assert r["rows_ms"] > 0 and r["proj_ms"] > 0 and r["reduce_ms"] >= 0
+    assert d.timing_read()["launches"] == 0
+    assert relerr(Y.cpu().numpy().astype(np.float64), orc.matmat(c1f, X)) <= TOL["bf16x2"]
+    d.close()
