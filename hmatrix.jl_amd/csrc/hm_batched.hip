@@ -29,6 +29,7 @@
 // leaves FP32, split in registers, same 3 MFMAs), HM_BF16 (dense leaves and
 // factors BF16; X still split, 2 MFMAs).
 #include <algorithm>
+#include <type_traits>
 
 #include "hm_internal.h"
 
@@ -39,6 +40,12 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef short short8 __attribute__((ext_vector_type(8)));
 
 enum : int { MODE_F32 = 1, MODE_BF16 = 2, MODE_BF16X2 = 3 };
+
+// K-steps a row-pass wavefront loads before their MFMAs.
+#ifndef HM_SKC
+#define HM_SKC 2
+#endif
+constexpr int SKC = HM_SKC;
 
 __device__ __forceinline__ uint32_t f2bf_bits(float f) {
     uint32_t u = __float_as_uint(f);
@@ -253,255 +260,278 @@ __global__ __launch_bounds__(256) void kb_reduce(const BRed *__restrict__ red,
 }
 
 // ---------------------------------------------------------------------------
-// row owner
-constexpr int BK_LD = KSTAGE + 8;   // [plane][rhs][k] BF16 stride: conflict-free b128 reads
+// row pass: one wavefront per strip of SROWS rows x NB right-hand sides.
+// No LDS and no barriers: every operand comes from global memory straight
+// into the MFMA registers.  Per K-step a lane loads its 8 A values of strip
+// row r16 (octet g) and, for each right-hand-side block h, the 8 B values
+// of k-rows [8g, 8g + 8) in column 16h + r16: X rows are contiguous in a
+// column of X, panel rows are gathered from t.  KC K-steps are loaded
+// together before their MFMAs.
 
-__device__ __forceinline__ int64_t w64(const uint32_t *w) {
-    return static_cast<int64_t>(static_cast<uint64_t>(w[0]) | (static_cast<uint64_t>(w[1]) << 32));
+// lane's 8 A values of one K-step as BF16 hi / lo vectors
+template <int MODE>
+__device__ __forceinline__ void load_a(const void *frag, const float *zeros, bool on, uint32_t afrag,
+                                       int lane_unit, int lo_units, bf16x8 &ah, bf16x8 &al) {
+    if (MODE == MODE_F32) {
+        const float *fp = on ? static_cast<const float *>(frag) + 4 * (static_cast<int64_t>(afrag) + lane_unit)
+                             : zeros;
+        const float4 a = reinterpret_cast<const float4 *>(fp)[0];
+        const float4 b = reinterpret_cast<const float4 *>(fp)[1];
+        float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        split8hw(v, ah, al);
+    } else {
+        const uint16_t *fb = static_cast<const uint16_t *>(frag) + 8 * (static_cast<int64_t>(afrag) + lane_unit);
+        const uint16_t *zb = reinterpret_cast<const uint16_t *>(zeros);
+        ah = load_bf8(on ? fb : zb);
+        if (MODE == MODE_BF16X2) al = load_bf8(on ? fb + 8 * lo_units : zb);
+        else al = __builtin_bit_cast(bf16x8, short8{0, 0, 0, 0, 0, 0, 0, 0});
+    }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void kb_rows(int64_t ntiles, int64_t wg_per_xcd,
-                                               const uint32_t *__restrict__ blobs,
-                                               const uint32_t *__restrict__ blobx,
-                                               const void *__restrict__ frag,
-                                               const float *__restrict__ zeros,
-                                               const float *__restrict__ T,
-                                               const float *__restrict__ X, int64_t ldx,
-                                               float *__restrict__ Y, int64_t ldy,
-                                               int ncol, float alpha, float beta, int beta_zero) {
-    // B rows of one staging batch as split BF16, [hi/lo][rhs][k row]; the
-    // same storage holds the tile's Y block (as floats) at the end.
-    __shared__ uint32_t bsm[2 * NB * BK_LD / 2];
-    __shared__ uint4 blob4[BLOB_MAX / 4];
-    uint16_t(*bs)[NB][BK_LD] = reinterpret_cast<uint16_t(*)[NB][BK_LD]>(bsm);
-    const uint32_t *blob = reinterpret_cast<const uint32_t *>(blob4);
+template <int MODE, int KC>
+__global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per_xcd,
+                                                 const uint32_t *__restrict__ soff,
+                                                 const uint32_t *__restrict__ recs,
+                                                 const int32_t *__restrict__ wgx,
+                                                 const void *__restrict__ frag,
+                                                 const float *__restrict__ zeros,
+                                                 const float *__restrict__ T,
+                                                 const float *__restrict__ X, int64_t ldx, int64_t ncols,
+                                                 float *__restrict__ Y, int64_t ldy,
+                                                 int ncol, float alpha, float beta, int beta_zero) {
+    // X window of the workgroup as FP32 [column of X][row], then 8 zeros
+    // that lanes without data read
+    __shared__ float xs[NB * XS_LD + 8];
     // Hardware places workgroup b on XCD b % 8; give each XCD a contiguous
-    // range of tiles so neighbouring tiles share X rows in that L2.
-    const int64_t tile = static_cast<int64_t>(blockIdx.x % 8) * wg_per_xcd + blockIdx.x / 8;
-    if (tile >= ntiles) return;                       // uniform across the workgroup
-    // the tile's descriptor blob: one round trip (two when it overflows)
-    if (threadIdx.x < BLOB_STRIDE / 4)
-        blob4[threadIdx.x] = reinterpret_cast<const uint4 *>(blobs + tile * BLOB_STRIDE)[threadIdx.x];
-    __syncthreads();
-    if (blob[7] != 0u) {
-        const int nw = static_cast<int>(blob[7]);
-        const uint4 *src = reinterpret_cast<const uint4 *>(blobx + w64(blob + 8));
-        __syncthreads();
-        for (int i = threadIdx.x; i < nw / 4; i += 256) blob4[i] = src[i];
-        __syncthreads();
-    }
-    const int64_t row0 = w64(blob);
-    const int R = static_cast<int>(blob[2]);
-    const int nbatch = static_cast<int>(blob[3]);
-    const int nseg = static_cast<int>(blob[4]);
-    const int nks = static_cast<int>(blob[5]);
-    const uint32_t *bat = blob + BLOB_HDR;
-    const uint32_t *sgw = bat + 4 * nbatch;
-    const uint32_t *ksw = sgw + 6 * nseg;
-    const uint32_t *tof = ksw + 4 * nks;
-
-    const int wave = threadIdx.x >> 6;               // M-tile of this wavefront
+    // range of strips so neighbouring strips share X rows in that L2.
+    const int64_t wg = static_cast<int64_t>(blockIdx.x % 8) * wg_per_xcd + blockIdx.x / 8;
+    if (wg * SWAVES >= nstrips) return;       // whole workgroup
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int64_t s = wg * SWAVES + wave;
     const int l = threadIdx.x & 63;
     const int r16 = l & 15;
     const int g = l >> 4;
-    const int Rm = min(16, R - 16 * wave);            // <= 0: wave has no rows
-    constexpr int planes = MODE == MODE_BF16X2 ? 2 : 1;
-    const bf16x8 zero8 = __builtin_bit_cast(bf16x8, short8{0, 0, 0, 0, 0, 0, 0, 0});
-    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    const int cmin = wgx[2 * wg], W = wgx[2 * wg + 1];
+    const float *xzero = xs + NB * XS_LD;
 
-    for (int bi = 0; bi < nbatch; ++bi) {
-        const int sb = static_cast<int>(bat[4 * bi]), se = static_cast<int>(bat[4 * bi + 1]);
-        const int kb = static_cast<int>(bat[4 * bi + 2]), ke = static_cast<int>(bat[4 * bi + 3]);
-        // (1) this wave's A fragments for every K-step of the batch.  Every
-        //     lane issues its loads unconditionally (lanes without data read
-        //     the zero buffer), so all of them are in flight together.
-        bf16x8 ah[KSLOTS], al[KSLOTS];
-        int krow[KSLOTS];
+    // Stage X rows [cmin, cmin + W) of all NB columns (zeros past ncols and
+    // ncol): (column, 4-row quad) units, a wavefront reading one column's
+    // rows contiguously; every load issued before the first LDS store.
+    auto stage = [&]() {
+        constexpr int NQ = XS_MAX / 4;                 // quads per column slot
+        constexpr int U = NB * NQ / 256;               // units per thread
+        f4u v[U];
+        int part[U];
 #pragma unroll
-        for (int u = 0; u < KSLOTS; ++u) {
-            const bool slot = kb + u < ke;
-            const uint32_t *kw = ksw + 4 * (slot ? kb + u : 0);
-            krow[u] = slot ? static_cast<int>(kw[2]) : 0;
-            const int nOct = slot ? static_cast<int>(kw[3]) : 0;
-            const bool on = slot && Rm > 0 && r16 < Rm && g < nOct;
-            const int64_t la = w64(kw) + static_cast<int64_t>(planes) * 128 * nOct * wave +
-                               static_cast<int64_t>(g * Rm + r16) * 8;
-            if (MODE == MODE_F32) {
-                const float *fp = on ? static_cast<const float *>(frag) + la : zeros;
-                const float4 a = reinterpret_cast<const float4 *>(fp)[0];
-                const float4 b = reinterpret_cast<const float4 *>(fp)[1];
-                float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-                split8hw(v, ah[u], al[u]);
-            } else {
-                const uint16_t *fb = static_cast<const uint16_t *>(frag);
-                const uint16_t *zb = reinterpret_cast<const uint16_t *>(zeros);
-                ah[u] = load_bf8(on ? fb + la : zb);
-                if (MODE == MODE_BF16X2)
-                    al[u] = load_bf8(on ? fb + la + static_cast<int64_t>(nOct) * Rm * 8 : zb);
-                else
-                    al[u] = zero8;
+        for (int u = 0; u < U; ++u) {
+            const int p = threadIdx.x + 256 * u;
+            const int c = p / NQ, q = p % NQ;
+            const int64_t row = static_cast<int64_t>(cmin) + 4 * q;
+            const bool in = c < ncol && 4 * q < W;
+            part[u] = in ? static_cast<int>(min<int64_t>(4, ncols - row)) : 0;
+            const float *src = part[u] == 4 ? X + static_cast<int64_t>(c) * ldx + row : zeros;
+            v[u] = *reinterpret_cast<const f4u *>(src);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (part[u] > 0 && part[u] < 4) {        // the quad cut by the last row of X
+                const int p = threadIdx.x + 256 * u;
+                const float *src = X + static_cast<int64_t>(p / NQ) * ldx + cmin + 4 * (p % NQ);
+                f4u w = {0.f, 0.f, 0.f, 0.f};
+                for (int e = 0; e < part[u]; ++e) w[e] = src[e];
+                v[u] = w;
             }
         }
-        // (2) stage the batch's B rows as split BF16 into bs[plane][rhs][k].
-        //     X rows (column-major X): (column, 4-row quad) units, one 16-byte
-        //     load each, a wavefront reading contiguous column segments.
-        //     t vectors: (row pair, 4-rhs quad) units, two 16-byte loads each.
-        //     Every load of a thread is issued before its first LDS store.
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = threadIdx.x + 256 * u;
+            *reinterpret_cast<float4 *>(&xs[(p / NQ) * XS_LD + 4 * (p % NQ)]) =
+                make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
+        }
+        if (threadIdx.x < 8) xs[NB * XS_LD + threadIdx.x] = 0.0f;
+    };
+
+    auto body = [&](auto staged) {
+    constexpr bool STAGED = decltype(staged)::value;
+    if (s >= nstrips) return;                 // this wavefront has no strip
+    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    const bool rok0 = r16 < ncol, rok1 = 16 + r16 < ncol;
+    const float *xc0 = X + static_cast<int64_t>(r16) * ldx;
+    const float *xc1 = X + static_cast<int64_t>(16 + r16) * ldx;
+    int row0 = 0, R = 0;
+    uint32_t rec = soff[s];
+    do {
+    const uint32_t *rw = recs + rec;
+    // The first 128 words of the record in two vector registers (lane l
+    // holds words l and 64 + l; the record arena is padded so these loads
+    // stay in bounds): header and descriptors are then read with readlane,
+    // one memory round trip for all of them.
+    const uint32_t hv0 = rw[l], hv1 = rw[64 + l];
+    auto word = [&](int i) -> uint32_t {          // i wave-uniform, < 128
+        const uint32_t a = __builtin_amdgcn_readlane(hv0, i & 63);
+        const uint32_t b = __builtin_amdgcn_readlane(hv1, i & 63);
+        return i < 64 ? a : b;
+    };
+    row0 = static_cast<int>(word(0));
+    R = static_cast<int>(word(1));
+    const int nd = static_cast<int>(word(2));        // dense K-steps
+    const int np = static_cast<int>(word(3));        // panel K-steps
+    const uint32_t pidx = word(4);                   // first t-row index block
+    auto desc = [&](int k) {                         // K-step k (dense first, then panel)
+        const int i = SREC_HDR + 4 * k;
+        return uint4{word(i), word(i + 1), word(i + 2), 0u};
+    };
+
+    // MFMAs of one K-step: 8 B values per lane and block (hi/lo split here)
+    auto kstep = [&](const bf16x8 &ah, const bf16x8 &al, const f4u (&b)[4]) {
+        bf16x8 xh0, xl0, xh1, xl1;
         {
-            constexpr int NQ = KSTAGE / 4;                 // row quads of the stage
-            constexpr int UX = NQ * NB / 256;              // X units per thread
-            constexpr int UT = (KSTAGE / 2) * (NB / 4) / 256;   // t units per thread
-            f4u xv[UX];
-            int xq[UX], xpart[UX];
-            const float *xsrc[UX];
+            const float v[8] = {b[0][0], b[0][1], b[0][2], b[0][3], b[1][0], b[1][1], b[1][2], b[1][3]};
+            split8hw(v, xh0, xl0);
+        }
+        {
+            const float v[8] = {b[2][0], b[2][1], b[2][2], b[2][3], b[3][0], b[3][1], b[3][2], b[3][3]};
+            split8hw(v, xh1, xl1);
+        }
+        acc0 = mfma(ah, xh0, acc0);
+        acc0 = mfma(ah, xl0, acc0);
+        acc1 = mfma(ah, xh1, acc1);
+        acc1 = mfma(ah, xl1, acc1);
+        if (MODE != MODE_BF16) {
+            acc0 = mfma(al, xh0, acc0);
+            acc1 = mfma(al, xh1, acc1);
+        }
+    };
+    // Panel K-step q: A fragments (on == false: the zero buffer) and the t
+    // rows of this lane's octet, indices [8g, 8g + 8) of the 32-word block
+    // at word `ib` of the record.  Every strip has at least one index block
+    // (rows of zeros when it has no panel), so the loads issued never depend
+    // on the strip: the wait counters then never over-wait on a branch.
+    auto load_panel_a = [&](int q, bool on, bf16x8 &ah, bf16x8 &al) {
+        const uint4 kd = desc(nd + (on ? q : 0));
+        const uint32_t afrag = kd.x, info = kd.z;
+        const int rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
+        load_a<MODE>(frag, zeros, on && g < nOct && r16 < rows, afrag,
+                     (g * rows + r16) * (MODE == MODE_F32 ? 2 : 1), nOct * rows, ah, al);
+    };
+    auto gather_t = [&](const uint4 &q0, const uint4 &q1, f4u (&b)[4]) {
+        const uint32_t ix[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
 #pragma unroll
-            for (int u = 0; u < UX; ++u) {
-                const int p = threadIdx.x + 256 * u;
-                const int q = p % NQ;
-                const int c = p / NQ;
-                const int r = 4 * q;
-                xq[u] = -1;
-                xpart[u] = 0;
-                const float *src = zeros;
-                for (int qq = sb; qq < se; ++qq) {
-                    const uint32_t *sw = sgw + 6 * qq;
-                    const int rr = r - static_cast<int>(sw[4]);
-                    const int klen = static_cast<int>(sw[2]);
-                    if (sw[3] == 0u && rr >= 0 && rr < ((klen + 31) & ~31)) {
-                        xq[u] = 1;
-                        if (c < ncol && rr < klen) {
-                            src = X + static_cast<int64_t>(c) * ldx + w64(sw) + rr;
-                            xpart[u] = min(4, klen - rr);    // valid rows of the quad
-                        }
-                    }
-                }
-                xsrc[u] = xpart[u] == 4 ? src : zeros;
-                xv[u] = *reinterpret_cast<const f4u *>(xsrc[u]);
-                xsrc[u] = src;
-            }
-            float4 ta[UT], tb[UT];
-            int tr[UT];
+        for (int e = 0; e < 8; ++e) {
+            const float *tr = T + static_cast<int64_t>(ix[e]) * NB + r16;
+            b[e >> 2][e & 3] = tr[0];
+            b[2 + (e >> 2)][e & 3] = tr[16];
+        }
+    };
+
+    // One batch of KC dense K-steps (the first batch also carries the first
+    // panel K-step): all loads first, then the MFMAs.
+    auto batch = [&](int k0, auto first) {
+        constexpr bool FIRST = decltype(first)::value;
+        uint4 q0, q1;
+        bf16x8 pah, pal;
+        if constexpr (FIRST) {
+            const uint4 *ip = reinterpret_cast<const uint4 *>(rw + pidx + 8 * g);
+            q0 = ip[0];
+            q1 = ip[1];
+            load_panel_a(0, np > 0, pah, pal);
+        }
+        bf16x8 ah[KC], al[KC];
+        f4u b[KC][4];       // [h * 2 + half]: B values 8g .. 8g+7 of block h
+        uint4 kd[KC];
 #pragma unroll
-            for (int u = 0; u < UT; ++u) {
-                const int p = threadIdx.x + 256 * u;
-                const int cq = p % (NB / 4);
-                const int r = 2 * (p / (NB / 4));
-                tr[u] = -1;
-                const float *sa = zeros, *sb2 = zeros;
-                for (int qq = sb; qq < se; ++qq) {
-                    const uint32_t *sw = sgw + 6 * qq;
-                    const int rr = r - static_cast<int>(sw[4]);
-                    const int klen = static_cast<int>(sw[2]);
-                    if (sw[3] == 1u && rr >= 0 && rr < ((klen + 31) & ~31)) {
-                        tr[u] = 1;
-                        const uint32_t *tt = tof + sw[5] + rr;
-                        if (rr < klen) sa = T + static_cast<int64_t>(tt[0]) * NB + 4 * cq;
-                        if (rr + 1 < klen) sb2 = T + static_cast<int64_t>(tt[1]) * NB + 4 * cq;
-                    }
-                }
-                ta[u] = *reinterpret_cast<const float4 *>(sa);
-                tb[u] = *reinterpret_cast<const float4 *>(sb2);
-            }
-            // quads cut by a segment end: element loads, after every vector
-            // load above has been issued
+        for (int u = 0; u < KC; ++u)
+            kd[u] = desc(max(0, min(k0 + u, nd - 1)));
 #pragma unroll
-            for (int u = 0; u < UX; ++u) {
-                if (xpart[u] > 0 && xpart[u] < 4) {
-                    f4u v = {0.f, 0.f, 0.f, 0.f};
-                    for (int e = 0; e < xpart[u]; ++e) v[e] = xsrc[u][e];
-                    xv[u] = v;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < UX; ++u) {
-                const int p = threadIdx.x + 256 * u;
-                const int q = p % NQ;
-                const int c = p / NQ;
-                uint2 hi, lo;
-                split2(xv[u][0], xv[u][1], hi.x, lo.x);
-                split2(xv[u][2], xv[u][3], hi.y, lo.y);
-                if (xq[u] > 0) {
-                    *reinterpret_cast<uint2 *>(&bs[0][c][4 * q]) = hi;
-                    *reinterpret_cast<uint2 *>(&bs[1][c][4 * q]) = lo;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < UT; ++u) {
-                const int p = threadIdx.x + 256 * u;
-                const int cq = p % (NB / 4);
-                const int r = 2 * (p / (NB / 4));
-                if (tr[u] > 0) {
-                    const float a4[4] = {ta[u].x, ta[u].y, ta[u].z, ta[u].w};
-                    const float b4[4] = {tb[u].x, tb[u].y, tb[u].z, tb[u].w};
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        uint32_t hi, lo;
-                        split2(a4[e], b4[e], hi, lo);
-                        *reinterpret_cast<uint32_t *>(&bs[0][4 * cq + e][r]) = hi;
-                        *reinterpret_cast<uint32_t *>(&bs[1][4 * cq + e][r]) = lo;
-                    }
-                }
+        for (int u = 0; u < KC; ++u) {
+            const bool live = k0 + u < nd;                          // wave-uniform
+            const uint32_t afrag = kd[u].x, src = kd[u].y, info = kd[u].z;
+            const int roff = info & 15, rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
+            const bool aon = live && g < nOct && r16 >= roff && r16 < roff + rows;
+            load_a<MODE>(frag, zeros, aon, afrag, (g * rows + r16 - roff) * (MODE == MODE_F32 ? 2 : 1),
+                         nOct * rows, ah[u], al[u]);
+            const bool on = live && g < nOct;
+            if constexpr (STAGED) {
+                // B rows from the LDS window (16-byte aligned reads)
+                const int off = static_cast<int>(src) - cmin + 8 * g;
+                const float *p0 = on ? xs + r16 * XS_LD + off : xzero;
+                const float *p1 = on ? xs + (16 + r16) * XS_LD + off : xzero;
+                const float4 v0 = reinterpret_cast<const float4 *>(p0)[0];
+                const float4 v1 = reinterpret_cast<const float4 *>(p0)[1];
+                const float4 v2 = reinterpret_cast<const float4 *>(p1)[0];
+                const float4 v3 = reinterpret_cast<const float4 *>(p1)[1];
+                b[u][0] = f4u{v0.x, v0.y, v0.z, v0.w};
+                b[u][1] = f4u{v1.x, v1.y, v1.z, v1.w};
+                b[u][2] = f4u{v2.x, v2.y, v2.z, v2.w};
+                b[u][3] = f4u{v3.x, v3.y, v3.z, v3.w};
+            } else {
+                const int64_t off = static_cast<int64_t>(src) + 8 * g;
+                const float *p0 = on && rok0 ? xc0 + off : zeros;
+                const float *p1 = on && rok1 ? xc1 + off : zeros;
+                b[u][0] = reinterpret_cast<const f4u *>(p0)[0];
+                b[u][1] = reinterpret_cast<const f4u *>(p0)[1];
+                b[u][2] = reinterpret_cast<const f4u *>(p1)[0];
+                b[u][3] = reinterpret_cast<const f4u *>(p1)[1];
             }
         }
-        __syncthreads();
-        // (3) MFMA over the batch's K-steps, B fragments straight from LDS
-        if (Rm > 0) {
-#pragma unroll
-            for (int u = 0; u < KSLOTS; ++u) {
-                if (kb + u < ke) {
-                    const int k0 = krow[u] + 8 * g;
-                    const bf16x8 xh0 = load_bf8(&bs[0][r16][k0]);
-                    const bf16x8 xl0 = load_bf8(&bs[1][r16][k0]);
-                    const bf16x8 xh1 = load_bf8(&bs[0][16 + r16][k0]);
-                    const bf16x8 xl1 = load_bf8(&bs[1][16 + r16][k0]);
-                    acc0 = mfma(ah[u], xh0, acc0);
-                    acc0 = mfma(ah[u], xl0, acc0);
-                    acc1 = mfma(ah[u], xh1, acc1);
-                    acc1 = mfma(ah[u], xl1, acc1);
-                    if (MODE != MODE_BF16) {
-                        acc0 = mfma(al[u], xh0, acc0);
-                        acc1 = mfma(al[u], xh1, acc1);
-                    }
-                }
-            }
+        f4u pb[4];
+        if constexpr (FIRST) {
+            // keep the gather behind the loads above: its addresses wait on
+            // the index loads
+            __builtin_amdgcn_sched_barrier(0);
+            gather_t(q0, q1, pb);
         }
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        // K-steps past the strip's end hold zero operands; their MFMAs run
+        // anyway (a branch around them would let the compiler move their
+        // loads behind the first wait)
+#pragma unroll
+        for (int u = 0; u < KC; ++u) kstep(ah[u], al[u], b[u]);
+        if constexpr (FIRST) kstep(pah, pal, pb);
+    };
+    batch(0, std::true_type{});
+    for (int k0 = KC; k0 < nd; k0 += KC) batch(k0, std::false_type{});
+    for (int q = 1; q < np; ++q) {                 // further panel K-steps
+        bf16x8 ah, al;
+        f4u b[4];
+        load_panel_a(q, true, ah, al);
+        const uint4 *ip = reinterpret_cast<const uint4 *>(rw + desc(nd + q).y + 8 * g);
+        gather_t(ip[0], ip[1], b);
+        kstep(ah, al, b);
     }
-    // Y block through LDS: [rhs][row] floats, then (column, 4-row quad)
-    // units written with one 16-byte store each.
-    float(*ys)[68] = reinterpret_cast<float(*)[68]>(bsm);
-    if (Rm > 0) {
+    rec = word(5);
+    } while (rec != 0u);
+
+    // D layout: lane (g, r16) holds strip rows 4g .. 4g+3 of column 16h + r16
+    const int r = 4 * g;
+    if (r < R) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            ys[r16][16 * wave + 4 * g + i] = acc0[i];
-            ys[16 + r16][16 * wave + 4 * g + i] = acc1[i];
-        }
-    }
-    __syncthreads();
-    {
-        const int q = threadIdx.x & 15;      // row quad
-        for (int c = threadIdx.x >> 4; c < ncol; c += 16) {
-            const int r = 4 * q;
-            if (r < R) {
-                float *dst = Y + static_cast<int64_t>(c) * ldy + row0 + r;
-                const float4 a = *reinterpret_cast<const float4 *>(&ys[c][r]);
-                f4u v = {alpha * a.x, alpha * a.y, alpha * a.z, alpha * a.w};
-                if (r + 4 <= R) {
-                    if (!beta_zero) {
-                        const f4u o = *reinterpret_cast<const f4u *>(dst);
-                        v += beta * o;
-                    }
-                    *reinterpret_cast<f4u *>(dst) = v;
-                } else {
+        for (int h = 0; h < 2; ++h) {
+            const int c = 16 * h + r16;
+            if (c >= ncol) continue;
+            const floatx4 a = h == 0 ? acc0 : acc1;
+            float *dst = Y + static_cast<int64_t>(c) * ldy + row0 + r;
+            f4u v = {alpha * a[0], alpha * a[1], alpha * a[2], alpha * a[3]};
+            if (r + 4 <= R) {
+                if (!beta_zero) v += beta * *reinterpret_cast<const f4u *>(dst);
+                *reinterpret_cast<f4u *>(dst) = v;
+            } else {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (r + e < R) dst[e] = beta_zero ? v[e] : beta * dst[e] + v[e];
-                }
+                for (int e = 0; e < 4; ++e)
+                    if (r + e < R) dst[e] = beta_zero ? v[e] : beta * dst[e] + v[e];
             }
         }
+    }
+    };
+    if (W > 0) {
+        stage();
+        __syncthreads();
+        body(std::true_type{});
+    } else {
+        body(std::false_type{});
     }
 }
+
 
 template <int MODE>
 static hipError_t run_b(const Direction &d, const void *zeros, int ncol, const float *X,
@@ -524,12 +554,13 @@ static hipError_t run_b(const Direction &d, const void *zeros, int ncol, const f
         if (e != hipSuccess) return e;
     }
     if (ev) (void)hipEventRecord(ev[1], s);
-    if (b.ntiles > 0) {
-        const int64_t per = (b.ntiles + 7) / 8;
-        hipLaunchKernelGGL(kb_rows<MODE>, dim3(static_cast<unsigned>(8 * per)), dim3(256), 0, s,
-                           b.ntiles, per, b.blobs, b.blobx, b.frag, static_cast<const float *>(zeros),
-                           static_cast<const float *>(b.t),
-                           X, ldx, Y, ldy, ncol, alpha, beta, beta == 0.0f ? 1 : 0);
+    if (b.nstrips > 0) {
+        const int64_t nwg = (b.nstrips + SWAVES - 1) / SWAVES;
+        const int64_t per = (nwg + 7) / 8;        // workgroups per XCD
+        hipLaunchKernelGGL((kb_strips<MODE, SKC>), dim3(static_cast<unsigned>(8 * per)), dim3(64 * SWAVES), 0, s,
+                           b.nstrips, per, b.soff, b.recs, b.wgx, b.frag, static_cast<const float *>(zeros),
+                           static_cast<const float *>(b.t), X, ldx, d.ncols,
+                           Y, ldy, ncol, alpha, beta, beta == 0.0f ? 1 : 0);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

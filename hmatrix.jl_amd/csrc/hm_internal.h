@@ -75,56 +75,46 @@ constexpr int NB = 32;
 // Columns of X one projection workgroup stages in LDS.
 constexpr int PROJ_SUPER = 256;
 
-// k-rows of B one staging batch of the row-owner kernel holds in LDS.
-constexpr int KSTAGE = 128;
-// K-steps (32 k-rows each) per batch.
-constexpr int KSLOTS = KSTAGE / 32;
+// Row pass of the batched path: the rows are cut into strips of SROWS rows,
+// one wavefront per strip (MFMA M = 16).  A strip is a list of K-steps of
+// 32 columns; each K-step multiplies a block of A fragments (pre-packed in
+// the MFMA A-operand order) with 32 rows of B that the wavefront loads
+// straight from global memory: X rows [src, src + 32) for a dense piece, or
+// 32 gathered rows of t for the strip's panel of low-rank columns.
+constexpr int SROWS = 16;
+// Strips per workgroup (one wavefront each).
+constexpr int SWAVES = 4;
+// Widest X window (rows of X, all NB columns) a row-pass workgroup stages in
+// LDS, and the LDS row stride: XS_LD / 4 odd keeps the 16-byte reads of 8
+// consecutive lanes (8 columns of X) on distinct bank groups.
+constexpr int XS_MAX = 256;
+constexpr int XS_LD = XS_MAX + 4;
 
-// K segment of a batched row tile: an R x klen block of A (a dense leaf's
-// rows, or the tile's panel of low-rank columns) whose B rows are X rows
-// [src, src + klen) (kind 0) or the t vectors tidx[src .. src + klen)
-// (kind 1).  It is staged at LDS row `lrow` of its batch.
-struct BSeg {
-    int64_t src;
-    int32_t klen;
-    int32_t kind;
-    int32_t lrow;
-    int32_t pad;
-};
-
-// One K-step (32 k values) of A fragments: all M-tiles at `frag`; B rows at
-// LDS row `krow`; nOct = number of stored 8-value octets (4, or fewer in a
-// segment's last K-step).
-struct BKs {
-    int64_t frag;
-    int32_t krow;
-    int32_t nOct;
-};
-
-// Staging batch: segments [sb, se) staged together, K-steps [kb, ke).
-struct BBatch {
-    int32_t sb, se, kb, ke;
-};
-
-struct BTile {
-    int64_t row0;
-    int32_t rows;
-    int32_t bb, be;    // batch range
-    int32_t pad;
-};
-
-// Per-tile descriptor blob read by the row-owner kernel in one round trip:
-// BLOB_STRIDE 32-bit words per tile.  Layout (words):
-//   0-1 row0, 2 rows, 3 nbatch, 4 nseg, 5 nks, 6 ntoff,
-//   7 words of the full blob when it does not fit inline (0 otherwise),
-//   8-9 word offset of that full blob in the overflow arena,
-//   then from word BLOB_HDR: batches (4 words: sb, se, kb, ke; indices local
-//   to the tile), segments (6 words: src lo/hi, klen, kind, lrow, tbase),
-//   K-steps (4 words: frag lo/hi, krow, nOct), t offsets (1 word each, in
-//   units of NB floats; kind-1 segment row r uses toff[tbase + r]).
-constexpr int BLOB_STRIDE = 128;
-constexpr int BLOB_HDR = 16;
-constexpr int BLOB_MAX = 2048;      // words of the largest blob the kernel stages
+// Strip record, 32-bit words, 16-byte aligned:
+//   0 row0, 1 rows (<= 16), 2 nd = dense K-steps, 3 np = panel K-steps,
+//   4 word offset (from the record start) of the first t-row index block,
+//   5 word offset in the record arena of the strip's next record (0: none;
+//     a strip with more than SREC_MAXK K-steps continues there), 6-7 zero,
+//   then 4 words per K-step, the nd dense ones first:
+//     afrag  offset of the K-step's A fragments in 16-byte units;
+//     src    dense: first X row; panel: word offset (from the record start)
+//            of its 32 t-row indices (entries past the panel point at a
+//            zero row of t);
+//     info   roff | rows << 4 | nOct << 9 | kind << 12, where the A block
+//            covers strip rows [roff, roff + rows) and octets [0, nOct) of
+//            the 4 8-column octets (others are zero and not stored); a
+//            dense piece whose octets would reach past the last row of X
+//            starts up to 7 columns early instead, with zero A columns;
+//     zero;
+//   then the 32-word t-row index blocks of the panel K-steps (one block of
+//   zero-row indices when the record has no panel).  Header and
+//   descriptors fit the first 128 words; 128 zero words follow the last
+//   record: the row pass loads 128 words from each record start.
+constexpr int SREC_HDR = 8;
+constexpr int SREC_MAXK = (128 - SREC_HDR) / 4;
+// A fragments of one K-step: `planes` planes of nOct octets x rows x 8
+// values (lane (g, r) reads values [8g, 8g + 8) of strip row roff + r).
+constexpr int SK_KIND_PANEL = 1;
 
 // Projection work unit: columns [col0, col0 + rows) of X and the B slices
 // of items [ib, ie) (bytes [vbeg, vbeg + vlen) of the slice arena, split
@@ -163,17 +153,18 @@ struct BRed {
 
 struct Batched {
     bool ready = false;
-    int64_t ntiles = 0;
-    uint32_t *blobs = nullptr;    // ntiles x BLOB_STRIDE words (multi-batch tiles)
-    uint32_t *blobx = nullptr;    // overflow blobs
+    int64_t nstrips = 0;
+    uint32_t *soff = nullptr;     // word offset of each strip record
+    uint32_t *recs = nullptr;     // strip records
+    int32_t *wgx = nullptr;       // per workgroup: X window (first row, rows; 0 rows: not staged)
     PTile *ptiles = nullptr;
     int64_t nptiles = 0;
     PItem *pitems = nullptr;
     BRed *red = nullptr;
     int64_t nred = 0;
-    void *frag = nullptr;    // A fragments of every tile segment
+    void *frag = nullptr;    // A fragments of every strip K-step
     void *vslc = nullptr;    // V slices per super-tile
-    float *t = nullptr;      // [t (tLen) | partials (partLen)]
+    float *t = nullptr;      // [t (tLen) | partials (partLen) | NB zeros]
     int64_t tLen = 0, partLen = 0;
 };
 

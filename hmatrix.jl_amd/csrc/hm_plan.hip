@@ -193,13 +193,25 @@ void *upload_vec(hm::Plan &p, const std::vector<T> &v) {
     return v.empty() ? nullptr : static_cast<void *>(dev_upload(p, v));
 }
 
-// Batched (multi-RHS) layout of one direction.  rk factors come from `fac`
-// (balanced FP64 arena), dense leaves from `dense` at each leaf's offA.
-void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
-                   const std::vector<OB> &obs, const double *fac, const double *dense) {
+// Host image of the batched layout of one direction.
+struct BatchedHost {
+    int64_t nstrips = 0;
+    std::vector<uint32_t> soff, recs;
+    std::vector<int32_t> wgx;
+    std::vector<hm::PTile> pt;
+    std::vector<hm::PItem> pitems;
+    std::vector<hm::BRed> red;
+    Arena fr, vs;
+    int64_t tLen = 0, partLen = 0;
+};
+
+// Batched (multi-RHS) layout of one direction for storage `mode` (1, 2, 3).
+// rk factors come from `fac` (balanced FP64 arena), dense leaves from
+// `dense` at each leaf's offA.  Returns false when the matrix does not fit
+// the layout (the product then runs one right-hand side at a time).
+bool layout_batched(int mode, int64_t nrows, int64_t ncols, const std::vector<OB> &obs,
+                    const double *fac, const double *dense, BatchedHost &H) {
     using namespace hm;
-    const int mode = p.storage;           // 1, 2 or 3
-    Batched &B = d.bat;
     // t offsets
     std::vector<int64_t> tOff(obs.size(), -1);
     int64_t tLen = 0;
@@ -208,132 +220,7 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
             tOff[b] = tLen;
             tLen += obs[b].k * NB;
         }
-    // ---- row owner: tiles, K segments, A fragments
-    const auto tl = make_tiles(nrows, obs);
-    std::vector<int64_t> tileStart(tl.size());
-    for (size_t i = 0; i < tl.size(); ++i) tileStart[i] = tl[i].first;
-    std::vector<std::vector<std::pair<int32_t, int32_t>>> per(tl.size());   // (block, ro)
-    for (size_t b = 0; b < obs.size(); ++b) {
-        const OB &o = obs[b];
-        if (!contributes(o)) continue;
-        size_t t = static_cast<size_t>(std::lower_bound(tileStart.begin(), tileStart.end(), o.row0) - tileStart.begin());
-        for (; t < tl.size() && tl[t].first < o.row0 + o.m; ++t)
-            per[t].emplace_back(static_cast<int32_t>(b), static_cast<int32_t>(tl[t].first - o.row0));
-    }
-    Arena fr;
-    fr.mode = mode == 1 ? 1 : 2;
-    const int planes = mode == 3 ? 2 : 1;
-    std::vector<BTile> tiles(tl.size());
-    std::vector<BBatch> batches;
-    std::vector<BSeg> segs;
-    std::vector<BKs> ksteps;
-    std::vector<int64_t> tidx;
-    // Pack the R x klen block `val(r, kk)` as MFMA A fragments: for every
-    // K-step s and M-tile mt, `planes` planes of nOct octets x Rm rows x 8.
-    auto pack = [&](int R, int64_t klen, const std::function<double(int, int64_t)> &val) {
-        fr.align(8);
-        const int64_t base = fr.size();
-        const int nK = static_cast<int>((klen + 31) / 32);
-        for (int s = 0; s < nK; ++s) {
-            const int64_t rem = klen - 32 * s;
-            const int nOct = rem >= 32 ? 4 : static_cast<int>((rem + 7) / 8);
-            for (int mt = 0; mt * 16 < R; ++mt) {
-                const int Rm = std::min(16, R - 16 * mt);
-                for (int plane = 0; plane < planes; ++plane)
-                    for (int g = 0; g < nOct; ++g)
-                        for (int r = 0; r < Rm; ++r)
-                            for (int e = 0; e < 8; ++e) {
-                                const int64_t kk = 32 * s + 8 * g + e;
-                                const double v = kk < klen ? val(16 * mt + r, kk) : 0.0;
-                                if (plane == 0) fr.push(v);
-                                else fr.push_lo(v);
-                            }
-            }
-        }
-        return base;
-    };
-    struct PendingSeg { BSeg seg; int64_t frag; };
-    for (size_t t = 0; t < tl.size(); ++t) {
-        const int32_t R = tl[t].second;
-        BTile &T = tiles[t];
-        T.row0 = tl[t].first;
-        T.rows = R;
-        std::vector<PendingSeg> ps;
-        // dense leaves, cut into pieces of <= KSTAGE columns
-        for (const auto &pr : per[t]) {
-            const OB &o = obs[pr.first];
-            if (o.kind != KIND_DENSE) continue;
-            const int64_t ro = pr.second;
-            const double *C = dense + o.offA;
-            for (int64_t c0 = 0; c0 < o.n; c0 += KSTAGE) {
-                const int64_t kl = std::min<int64_t>(KSTAGE, o.n - c0);
-                PendingSeg q{};
-                q.seg.src = o.col0 + c0;
-                q.seg.klen = static_cast<int32_t>(kl);
-                q.seg.kind = 0;
-                q.frag = pack(R, kl, [&](int r, int64_t kk) { return C[(ro + r) + (c0 + kk) * o.m]; });
-                ps.push_back(q);
-            }
-        }
-        // low-rank leaves: the panel of their A columns, B rows = t vectors
-        std::vector<std::pair<const double *, int64_t>> ucols;   // column pointer (row ro), t offset
-        for (const auto &pr : per[t]) {
-            const OB &o = obs[pr.first];
-            if (o.kind != KIND_RK) continue;
-            for (int64_t j = 0; j < o.k; ++j)
-                ucols.emplace_back(fac + o.offA + j * o.m + pr.second, tOff[pr.first] + j * NB);
-        }
-        for (size_t c0 = 0; c0 < ucols.size(); c0 += KSTAGE) {
-            const int64_t kl = std::min<int64_t>(KSTAGE, static_cast<int64_t>(ucols.size() - c0));
-            PendingSeg q{};
-            q.seg.src = static_cast<int64_t>(tidx.size());
-            q.seg.klen = static_cast<int32_t>(kl);
-            q.seg.kind = 1;
-            for (int64_t j = 0; j < kl; ++j) tidx.push_back(ucols[c0 + j].second);
-            q.frag = pack(R, kl, [&](int r, int64_t kk) { return ucols[c0 + kk].first[r]; });
-            ps.push_back(q);
-        }
-        // batches of segments whose rounded k-rows fit the LDS stage
-        T.bb = static_cast<int32_t>(batches.size());
-        size_t i = 0;
-        while (i < ps.size()) {
-            BBatch bt{};
-            bt.sb = static_cast<int32_t>(segs.size());
-            bt.kb = static_cast<int32_t>(ksteps.size());
-            int32_t lrow = 0;
-            while (i < ps.size()) {
-                const int32_t k32 = (ps[i].seg.klen + 31) & ~31;   // K-steps start 32 rows apart
-                if (lrow + k32 > KSTAGE) break;
-                BSeg sg = ps[i].seg;
-                sg.lrow = lrow;
-                segs.push_back(sg);
-                const int nK = (sg.klen + 31) / 32;
-                for (int s = 0; s < nK; ++s) {
-                    const int rem = sg.klen - 32 * s;
-                    BKs ks{};
-                    ks.frag = ps[i].frag + static_cast<int64_t>(s) * planes * 32 * R;
-                    ks.krow = lrow + 32 * s;
-                    ks.nOct = rem >= 32 ? 4 : (rem + 7) / 8;
-                    ksteps.push_back(ks);
-                }
-                lrow += k32;
-                ++i;
-            }
-            bt.se = static_cast<int32_t>(segs.size());
-            bt.ke = static_cast<int32_t>(ksteps.size());
-            batches.push_back(bt);
-        }
-        T.be = static_cast<int32_t>(batches.size());
-    }
-    // ---- projection: super-tiles of PROJ_SUPER columns
-    const int64_t nst = (ncols + PROJ_SUPER - 1) / PROJ_SUPER;
-    std::vector<std::vector<int32_t>> perst(static_cast<size_t>(nst));
-    for (size_t b = 0; b < obs.size(); ++b) {
-        const OB &o = obs[b];
-        if (o.kind != KIND_RK || !contributes(o)) continue;
-        for (int64_t st = o.col0 / PROJ_SUPER; st <= (o.col0 + o.n - 1) / PROJ_SUPER; ++st)
-            perst[st].push_back(static_cast<int32_t>(b));
-    }
+    // chunked projections: partial sums of leaves wider than one super-tile
     std::vector<int64_t> pOff(obs.size(), -1), nch(obs.size(), 0), chunkIdx(obs.size(), 0);
     int64_t partLen = 0;
     std::vector<BRed> red;
@@ -346,6 +233,223 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
             partLen += nch[b] * o.k * NB;
             red.push_back(BRed{pOff[b], tOff[b], static_cast<int32_t>(nch[b]), static_cast<int32_t>(o.k * NB)});
         }
+    }
+    // ---- row pass: strips of SROWS rows, K-steps of 32 columns
+    // per-column path only: offsets past 2^31, or fewer than 8 columns (an
+    // octet of X rows must fit in X)
+    if (nrows >= (int64_t(1) << 31) || ncols >= (int64_t(1) << 31) || ncols < 8) return false;
+    const int64_t nstrips = (nrows + SROWS - 1) / SROWS;
+    struct Piece {            // rows [roff, roff + rows) of a strip x columns [c0, c0 + n)
+        int32_t roff, rows;
+        int64_t c0, n;
+        std::vector<std::pair<int32_t, int64_t>> parts;   // (leaf, first column of the leaf part)
+    };
+    std::vector<std::vector<Piece>> dp(static_cast<size_t>(nstrips));
+    std::vector<std::vector<int32_t>> rp(static_cast<size_t>(nstrips));     // low-rank leaves
+    for (size_t b = 0; b < obs.size(); ++b) {
+        const OB &o = obs[b];
+        if (!contributes(o)) continue;
+        for (int64_t t = o.row0 / SROWS; t * SROWS < o.row0 + o.m; ++t) {
+            if (o.kind == KIND_RK) {
+                rp[t].push_back(static_cast<int32_t>(b));
+                continue;
+            }
+            const int64_t r0 = std::max(o.row0, t * SROWS), r1 = std::min(o.row0 + o.m, (t + 1) * SROWS);
+            Piece pc;
+            pc.roff = static_cast<int32_t>(r0 - t * SROWS);
+            pc.rows = static_cast<int32_t>(r1 - r0);
+            pc.c0 = o.col0;
+            pc.n = o.n;
+            pc.parts.emplace_back(static_cast<int32_t>(b), 0);
+            dp[t].push_back(std::move(pc));
+        }
+    }
+    Arena fr;
+    fr.mode = mode == 1 ? 1 : 2;
+    const int planes = mode == 3 ? 2 : 1;
+    const int64_t lane_elems = 8;                  // values per lane and octet
+    const int64_t unit = mode == 1 ? 4 : 8;        // arena elements per 16 bytes
+    const int64_t zrow = (tLen + partLen) / NB;   // zero row of t (after the partials)
+    std::vector<uint32_t> recs, soff(static_cast<size_t>(nstrips));
+    // Pack one K-step of A fragments: rows [roff, roff + rows) of the strip,
+    // columns [32 s, 32 s + 32) of a block whose value is val(r, kk), klen
+    // valid columns.  Returns the offset in 16-byte units.
+    auto pack = [&](int rows, int s, int64_t klen, int nOct,
+                    const std::function<double(int, int64_t)> &val) -> uint32_t {
+        fr.align(unit);
+        const int64_t at = fr.size() / unit;
+        if (at >= (int64_t(1) << 32)) throw std::length_error("fragment arena exceeds 64 GiB");
+        for (int plane = 0; plane < planes; ++plane)
+            for (int g = 0; g < nOct; ++g)
+                for (int r = 0; r < rows; ++r)
+                    for (int e = 0; e < lane_elems; ++e) {
+                        const int64_t kk = 32 * s + 8 * g + e;
+                        const double v = kk < klen ? val(r, kk) : 0.0;
+                        if (plane == 0) fr.push(v);
+                        else fr.push_lo(v);
+                    }
+        return static_cast<uint32_t>(at);
+    };
+    // merge pieces with equal rows whose columns continue each other
+    std::vector<std::vector<Piece>> merged(static_cast<size_t>(nstrips));
+    for (int64_t t = 0; t < nstrips; ++t) {
+        std::vector<Piece> &ps = dp[t];
+        std::sort(ps.begin(), ps.end(), [](const Piece &x, const Piece &y) {
+            return std::tie(x.roff, x.rows, x.c0) < std::tie(y.roff, y.rows, y.c0);
+        });
+        std::vector<Piece> &mg = merged[t];
+        for (Piece &pc : ps) {
+            if (!mg.empty() && mg.back().roff == pc.roff && mg.back().rows == pc.rows &&
+                mg.back().c0 + mg.back().n == pc.c0) {
+                mg.back().parts.emplace_back(pc.parts[0].first, mg.back().n);
+                mg.back().n += pc.n;
+            } else {
+                mg.push_back(std::move(pc));
+            }
+        }
+        std::vector<Piece>().swap(ps);
+    }
+    // Workgroup X windows: the SWAVES strips of a workgroup read their dense
+    // pieces' X rows from one window [cmin, cmin + W) staged in LDS when the
+    // window (pieces started at a multiple of 4, rounded up to octets) is at
+    // most XS_MAX rows wide; otherwise (W = 0) straight from global memory.
+    const int64_t nwg = (nstrips + SWAVES - 1) / SWAVES;
+    std::vector<int32_t> wgx(static_cast<size_t>(2 * nwg), 0);
+    for (int64_t w = 0; w < nwg; ++w) {
+        int64_t cmin = INT64_MAX, cmax = -1;
+        for (int64_t t = w * SWAVES; t < std::min(nstrips, (w + 1) * SWAVES); ++t)
+            for (const Piece &pc : merged[t]) {
+                const int64_t a0 = pc.c0 & ~int64_t(3);
+                cmin = std::min(cmin, a0);
+                cmax = std::max(cmax, a0 + ((pc.c0 + pc.n - a0 + 7) & ~int64_t(7)));
+            }
+        if (cmax > cmin && cmax - cmin <= XS_MAX) {
+            wgx[2 * w] = static_cast<int32_t>(cmin);
+            wgx[2 * w + 1] = static_cast<int32_t>(cmax - cmin);
+        }
+    }
+    for (int64_t t = 0; t < nstrips; ++t) {
+        const int64_t srow0 = t * SROWS;
+        const int32_t R = static_cast<int32_t>(std::min<int64_t>(SROWS, nrows - srow0));
+        const bool staged = wgx[2 * (t / SWAVES) + 1] > 0;
+        const std::vector<Piece> &mg = merged[t];
+        std::vector<uint32_t> dsc, pan, idxw;     // dense / panel descriptors, panel index blocks
+        int32_t nd = 0, np = 0;
+        auto add_step = [&](std::vector<uint32_t> &to, uint32_t afrag, uint32_t src, int roff, int rows,
+                            int nOct, int kind) {
+            to.push_back(afrag);
+            to.push_back(src);
+            to.push_back(static_cast<uint32_t>(roff | (rows << 4) | (nOct << 9) | (kind << 12)));
+            to.push_back(0u);
+        };
+        // K-steps of X columns [x0, x0 + w) of one merged piece whose piece
+        // column kk sits at X column pc.c0 + kk; only piece columns
+        // [klo, pc.n) are taken, every other column holds zero A values.
+        auto emit = [&](const Piece &pc, int64_t x0, int64_t w, int64_t klo) {
+            auto val = [&](int r, int64_t j) {
+                const int64_t kk = x0 + j - pc.c0;
+                if (kk < klo || kk >= pc.n) return 0.0;
+                size_t q = pc.parts.size() - 1;
+                while (pc.parts[q].second > kk) --q;
+                const OB &o = obs[pc.parts[q].first];
+                const int64_t lr = srow0 + pc.roff + r - o.row0;
+                return dense[o.offA + lr + (kk - pc.parts[q].second) * o.m];
+            };
+            const int nK = static_cast<int>((w + 31) / 32);
+            for (int s = 0; s < nK; ++s) {
+                const int64_t rem = w - 32 * s;
+                const int nOct = rem >= 32 ? 4 : static_cast<int>((rem + 7) / 8);
+                const uint32_t af = pack(pc.rows, s, w, nOct, val);
+                add_step(dsc, af, static_cast<uint32_t>(x0 + 32 * s), pc.roff, pc.rows, nOct, 0);
+                ++nd;
+            }
+        };
+        for (const Piece &pc : mg) {
+            if (staged) {
+                // from the LDS window: start on a multiple of 4 (16-byte LDS
+                // reads); rows past ncols are zeros there
+                const int64_t a0 = pc.c0 & ~int64_t(3);
+                emit(pc, a0, pc.c0 + pc.n - a0, 0);
+                continue;
+            }
+            // Every octet of X rows a K-step reads must end by ncols: a piece
+            // whose rounded-up end passes ncols starts up to 7 columns early
+            // (zero A columns); if it cannot (it starts within those 7
+            // columns), its whole octets are one run and its last columns a
+            // second run over the final 8 columns of X.
+            const int64_t n8 = (pc.n + 7) & ~int64_t(7);
+            const int64_t over = pc.c0 + n8 - ncols;
+            if (over <= 0) {
+                emit(pc, pc.c0, pc.n, 0);
+            } else if (pc.c0 >= over) {
+                emit(pc, pc.c0 - over, pc.n + over, 0);
+            } else {
+                const int64_t whole = pc.n & ~int64_t(7);
+                if (whole > 0) emit(pc, pc.c0, whole, 0);
+                emit(pc, ncols - 8, 8, whole);
+            }
+        }
+        // the panel of low-rank columns: A rows of every leaf crossing the strip
+        std::vector<std::pair<int32_t, int64_t>> ucols;     // (leaf, rank column)
+        for (int32_t b : rp[t])
+            for (int64_t j = 0; j < obs[b].k; ++j) ucols.emplace_back(b, j);
+        const int64_t nu = static_cast<int64_t>(ucols.size());
+        for (int s = 0; 32 * s < nu; ++s) {
+            const int64_t rem = nu - 32 * s;
+            const int nOct = rem >= 32 ? 4 : static_cast<int>((rem + 7) / 8);
+            const uint32_t af = pack(R, s, nu, nOct, [&](int r, int64_t kk) {
+                const OB &o = obs[ucols[kk].first];
+                const int64_t lr = srow0 + r - o.row0;
+                return (lr >= 0 && lr < o.m) ? fac[o.offA + ucols[kk].second * o.m + lr] : 0.0;
+            });
+            add_step(pan, af, 0u, 0, R, nOct, SK_KIND_PANEL);
+            ++np;
+            for (int e = 0; e < 32; ++e) {
+                const int64_t kk = 32 * s + e;
+                const int64_t row = kk < nu ? (tOff[ucols[kk].first] + ucols[kk].second * NB) / NB : zrow;
+                idxw.push_back(static_cast<uint32_t>(row));
+            }
+        }
+        // Records of at most SREC_MAXK K-steps (dense first, then panel), each
+        // followed by the index blocks of its panel K-steps (a block of
+        // zero-row indices when it has none), chained through word 5.
+        const int32_t ntot = nd + np;
+        soff[t] = static_cast<uint32_t>(recs.size());
+        for (int32_t k0 = 0; k0 == 0 || k0 < ntot; k0 += SREC_MAXK) {
+            const int32_t k1 = std::min(ntot, k0 + SREC_MAXK);
+            const int32_t rd = std::max(0, std::min(nd, k1) - k0);     // dense steps in this record
+            const int32_t rpn = (k1 - k0) - rd;                          // panel steps
+            const int32_t p0 = std::max(0, k0 - nd);                     // first panel step
+            std::vector<uint32_t> rec(SREC_HDR, 0u);
+            rec.insert(rec.end(), dsc.begin() + 4 * std::min(k0, nd), dsc.begin() + 4 * std::min(k1, nd));
+            const size_t pd = rec.size();
+            rec.insert(rec.end(), pan.begin() + 4 * p0, pan.begin() + 4 * (p0 + rpn));
+            const size_t ib = rec.size();
+            for (int32_t q = 0; q < rpn; ++q) {
+                rec[pd + 4 * q + 1] = static_cast<uint32_t>(ib + 32 * q);
+                rec.insert(rec.end(), idxw.begin() + 32 * (p0 + q), idxw.begin() + 32 * (p0 + q + 1));
+            }
+            if (rpn == 0) rec.insert(rec.end(), 32, static_cast<uint32_t>(zrow));
+            rec[0] = static_cast<uint32_t>(srow0);
+            rec[1] = static_cast<uint32_t>(R);
+            rec[2] = static_cast<uint32_t>(rd);
+            rec[3] = static_cast<uint32_t>(rpn);
+            rec[4] = static_cast<uint32_t>(ib);
+            while (rec.size() % 4) rec.push_back(0u);
+            const size_t at = recs.size();
+            if (at + rec.size() >= (size_t(1) << 32)) throw std::length_error("strip records exceed 2^32 words");
+            if (k1 < ntot) rec[5] = static_cast<uint32_t>(at + rec.size());   // next record
+            recs.insert(recs.end(), rec.begin(), rec.end());
+        }
+    }
+    // ---- projection: super-tiles of PROJ_SUPER columns
+    const int64_t nst = (ncols + PROJ_SUPER - 1) / PROJ_SUPER;
+    std::vector<std::vector<int32_t>> perst(static_cast<size_t>(nst));
+    for (size_t b = 0; b < obs.size(); ++b) {
+        const OB &o = obs[b];
+        if (o.kind != KIND_RK || !contributes(o)) continue;
+        for (int64_t st = o.col0 / PROJ_SUPER; st <= (o.col0 + o.n - 1) / PROJ_SUPER; ++st)
+            perst[st].push_back(static_cast<int32_t>(b));
     }
     Arena vs;
     vs.mode = mode == 1 ? 1 : 2;      // split-BF16 slices: hi plane then lo plane
@@ -402,85 +506,44 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
         }
         close_unit();
     }
-    if (tl.size() >= (size_t(1) << 31) || ksteps.size() >= (size_t(1) << 31) || pitems.size() >= (size_t(1) << 31))
-        throw std::length_error("work lists exceed 2^31 entries");
-    // per-tile descriptor blobs (inline, or header + overflow blob)
-    std::vector<uint32_t> blobs, blobx;
-    for (size_t t = 0; t < tiles.size(); ++t) {
-        const BTile &T = tiles[t];
-        std::vector<uint32_t> w(BLOB_HDR, 0u);
-        auto put64 = [&](int64_t v) {
-            w.push_back(static_cast<uint32_t>(static_cast<uint64_t>(v) & 0xffffffffu));
-            w.push_back(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
-        };
-        const int32_t sb0 = T.be > T.bb ? batches[T.bb].sb : 0;
-        const int32_t kb0 = T.be > T.bb ? batches[T.bb].kb : 0;
-        const int32_t se1 = T.be > T.bb ? batches[T.be - 1].se : 0;
-        const int32_t ke1 = T.be > T.bb ? batches[T.be - 1].ke : 0;
-        for (int32_t b = T.bb; b < T.be; ++b) {
-            w.push_back(static_cast<uint32_t>(batches[b].sb - sb0));
-            w.push_back(static_cast<uint32_t>(batches[b].se - sb0));
-            w.push_back(static_cast<uint32_t>(batches[b].kb - kb0));
-            w.push_back(static_cast<uint32_t>(batches[b].ke - kb0));
-        }
-        std::vector<uint32_t> toff;
-        for (int32_t q = sb0; q < se1; ++q) {
-            const BSeg &sg = segs[q];
-            put64(sg.src);
-            w.push_back(static_cast<uint32_t>(sg.klen));
-            w.push_back(static_cast<uint32_t>(sg.kind));
-            w.push_back(static_cast<uint32_t>(sg.lrow));
-            w.push_back(static_cast<uint32_t>(toff.size()));
-            if (sg.kind == 1)
-                for (int32_t j = 0; j < sg.klen; ++j) {
-                    const int64_t to = tidx[sg.src + j];
-                    if (to % NB || to / NB >= (int64_t(1) << 32)) throw std::length_error("t offset out of range");
-                    toff.push_back(static_cast<uint32_t>(to / NB));
-                }
-        }
-        for (int32_t k = kb0; k < ke1; ++k) {
-            put64(ksteps[k].frag);
-            w.push_back(static_cast<uint32_t>(ksteps[k].krow));
-            w.push_back(static_cast<uint32_t>(ksteps[k].nOct));
-        }
-        w.insert(w.end(), toff.begin(), toff.end());
-        while (w.size() % 4) w.push_back(0u);
-        w[0] = static_cast<uint32_t>(static_cast<uint64_t>(T.row0) & 0xffffffffu);
-        w[1] = static_cast<uint32_t>(static_cast<uint64_t>(T.row0) >> 32);
-        w[2] = static_cast<uint32_t>(T.rows);
-        w[3] = static_cast<uint32_t>(T.be - T.bb);
-        w[4] = static_cast<uint32_t>(se1 - sb0);
-        w[5] = static_cast<uint32_t>(ke1 - kb0);
-        w[6] = static_cast<uint32_t>(toff.size());
-        w.resize(std::max(w.size(), static_cast<size_t>(BLOB_STRIDE)), 0u);
-        blobs.resize(blobs.size() + BLOB_STRIDE, 0u);
-        uint32_t *dst = blobs.data() + blobs.size() - BLOB_STRIDE;
-        if (w.size() == static_cast<size_t>(BLOB_STRIDE)) {
-            std::copy(w.begin(), w.end(), dst);
-        } else {
-            if (w.size() > static_cast<size_t>(BLOB_MAX)) throw std::length_error("tile descriptor exceeds BLOB_MAX");
-            while (blobx.size() % 4) blobx.push_back(0u);
-            const int64_t at = static_cast<int64_t>(blobx.size());
-            w[7] = static_cast<uint32_t>(w.size());
-            w[8] = static_cast<uint32_t>(static_cast<uint64_t>(at) & 0xffffffffu);
-            w[9] = static_cast<uint32_t>(static_cast<uint64_t>(at) >> 32);
-            blobx.insert(blobx.end(), w.begin(), w.end());
-            std::copy(w.begin(), w.begin() + BLOB_HDR, dst);
-        }
-    }
-    B.ntiles = static_cast<int64_t>(blobs.size() / BLOB_STRIDE);
-    B.blobs = dev_upload(p, blobs);
-    B.blobx = dev_upload(p, blobx);
-    B.ptiles = dev_upload(p, pt);
-    B.nptiles = static_cast<int64_t>(pt.size());
-    B.pitems = dev_upload(p, pitems);
-    B.red = dev_upload(p, red);
-    B.nred = static_cast<int64_t>(red.size());
-    B.frag = fr.mode == 1 ? upload_vec(p, fr.f) : upload_vec(p, fr.h);
-    B.vslc = vs.mode == 1 ? upload_vec(p, vs.f) : upload_vec(p, vs.h);
-    B.tLen = tLen;
-    B.partLen = partLen;
-    B.t = static_cast<float *>(dev_alloc(p, static_cast<size_t>(tLen + partLen) * sizeof(float)));
+    if (pitems.size() >= (size_t(1) << 31)) throw std::length_error("work lists exceed 2^31 entries");
+    recs.resize(recs.size() + 128, 0u);   // the row pass reads 128 words from every record start
+    H.nstrips = nstrips;
+    H.wgx = std::move(wgx);
+    H.soff = std::move(soff);
+    H.recs = std::move(recs);
+    H.pt = std::move(pt);
+    H.pitems = std::move(pitems);
+    H.red = std::move(red);
+    H.fr = std::move(fr);
+    H.vs = std::move(vs);
+    H.tLen = tLen;
+    H.partLen = partLen;
+    return true;
+}
+
+void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
+                   const std::vector<OB> &obs, const double *fac, const double *dense) {
+    using namespace hm;
+    BatchedHost H;
+    if (!layout_batched(p.storage, nrows, ncols, obs, fac, dense, H)) return;
+    Batched &B = d.bat;
+    B.nstrips = H.nstrips;
+    B.soff = dev_upload(p, H.soff);
+    B.recs = dev_upload(p, H.recs);
+    B.wgx = dev_upload(p, H.wgx);
+    B.ptiles = dev_upload(p, H.pt);
+    B.nptiles = static_cast<int64_t>(H.pt.size());
+    B.pitems = dev_upload(p, H.pitems);
+    B.red = dev_upload(p, H.red);
+    B.nred = static_cast<int64_t>(H.red.size());
+    B.frag = H.fr.mode == 1 ? upload_vec(p, H.fr.f) : upload_vec(p, H.fr.h);
+    B.vslc = H.vs.mode == 1 ? upload_vec(p, H.vs.f) : upload_vec(p, H.vs.h);
+    B.tLen = H.tLen;
+    B.partLen = H.partLen;
+    const size_t tsz = static_cast<size_t>(H.tLen + H.partLen + NB) * sizeof(float);
+    B.t = static_cast<float *>(dev_alloc(p, tsz));
+    hip_check(hipMemset(B.t, 0, tsz), "hipMemset");
     B.ready = true;
 }
 
@@ -915,6 +978,12 @@ int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
     } catch (const std::bad_alloc &) {
         free_plan(p.release());
         return fail(HM_ENOMEM, "host allocation failed");
+    } catch (const std::length_error &e) {
+        free_plan(p.release());
+        return fail(HM_EUNSUPPORTED, std::string("plan layout: ") + e.what());
+    } catch (const std::exception &e) {
+        free_plan(p.release());
+        return fail(HM_EINVAL, std::string("plan layout: ") + e.what());
     }
     *plan = reinterpret_cast<hm_plan_t>(p.release());
     return HM_OK;

@@ -52,3 +52,28 @@ def c4_members(hm, nmat=64, nlo=1000, nhi=40000, seed=6):
     """Uneven batch of independent H-matrices (see hmatrix_amd.workloads)."""
     from hmatrix_amd.workloads import uneven_batch
     return uneven_batch(hm, nmat=nmat, nlo=nlo, nhi=nhi, seed=seed)
+
+
+def comb_flat(hm, seed: int, nrows: int = 41, nblk: int = 70):
+    """A block table with many narrow leaves per row: column blocks of width
+    3..12, each cut at its own row into two leaves (dense or low rank, rank
+    up to 6), so a 16-row strip crosses well over a hundred leaves."""
+    rng = np.random.default_rng(seed)
+    widths = rng.integers(3, 13, size=nblk)
+    blocks, chunks, off, c0 = [], [], 0, 0
+    for w in widths:
+        cut = int(rng.integers(1, nrows))
+        for r0, m in ((0, cut), (cut, nrows - cut)):
+            if rng.random() < 0.5:
+                C = rng.standard_normal((m, w))
+                blocks.append((0, r0, c0, m, w, 0, off, 0))
+                chunks.append(C.ravel(order="F"))
+                off += C.size
+            else:
+                k = int(rng.integers(1, 7))
+                A, B = rng.standard_normal((m, k)), rng.standard_normal((w, k))
+                blocks.append((1, r0, c0, m, w, k, off, off + A.size))
+                chunks += [A.ravel(order="F"), B.ravel(order="F")]
+                off += A.size + B.size
+        c0 += int(w)
+    return hm.FlatHmat(nrows, c0, np.array(blocks, dtype=np.int64), np.concatenate(chunks))

@@ -100,3 +100,17 @@ def test_kernel_timing_counts_launches(hm, orc, c1f):
     assert d.timing_read()["launches"] == 0
     assert relerr(Y.cpu().numpy().astype(np.float64), orc.matmat(c1f, X)) <= TOL["bf16x2"]
     d.close()
+
+
+@pytest.mark.parametrize("dtype", ["bf16x2", "f32", "bf16"])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_many_leaves_per_strip(hm, orc, dtype, seed):
+    """More than a hundred narrow leaves cross each 16-row strip: the row
+    pass walks several chained descriptor records per strip."""
+    from tests.helpers import comb_flat
+    f = comb_flat(hm, seed)
+    X = np.random.default_rng(seed).standard_normal((f.ncols, 32))
+    assert relerr(_run(hm, f, dtype, X), orc.matmat(f, X)) <= TOL[dtype]
+    V = np.random.default_rng(seed + 5).standard_normal((f.nrows, 32))
+    ref = np.stack([orc.matvec_t(f, V[:, i]) for i in range(32)], 1)
+    assert relerr(_run(hm, f, dtype, V, transpose=True), ref) <= TOL[dtype]
