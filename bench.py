@@ -300,15 +300,15 @@ def main() -> int:
                    "rk_leaves": f.counts()[1], "replicas": R},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(args.storage, "kb_rows"),
-                     "kernel": "kb_rows (batched row pass)", "algorithmic_bytes": kb["rows"],
+                     "traffic": profiled_traffic(args.storage, "kb_strips"),
+                     "kernel": "kb_strips (batched row pass)", "algorithmic_bytes": kb["rows"],
                      "us_per_launch": kus["rows"]},
         "kernels": {"kb_proj": {"us": kus["proj"], "algorithmic_bytes": kb["proj"], "GB/s": kgbs["proj"],
                                 "traffic": profiled_traffic(args.storage, "kb_proj")},
                     "kb_reduce": {"us": kus["reduce"], "algorithmic_bytes": 0,
                                   "traffic": profiled_traffic(args.storage, "kb_reduce")},
-                    "kb_rows": {"us": kus["rows"], "algorithmic_bytes": kb["rows"], "GB/s": kgbs["rows"],
-                                "traffic": profiled_traffic(args.storage, "kb_rows")}},
+                    "kb_strips": {"us": kus["rows"], "algorithmic_bytes": kb["rows"], "GB/s": kgbs["rows"],
+                                  "traffic": profiled_traffic(args.storage, "kb_strips")}},
         "ms_per_step_with_kernel_events": ms_ev,
         "step_GBs": step_gbs,
         "step_frac": step_gbs / HBM_PEAK_GBS,

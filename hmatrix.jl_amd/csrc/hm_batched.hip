@@ -2,28 +2,29 @@
 // up to NB = 32 right-hand sides, FP32 accumulation, on MFMA.
 //
 // Three kernels per panel, each streaming its factors exactly once:
-//   kb_proj   t_b = B_b' X[cols(b)] for every low-rank leaf.  One workgroup
-//             per 256-column super-tile stages X[super-tile] in LDS as split
-//             BF16 (hi + lo); its 4 wavefronts walk the leaves crossing the
-//             super-tile, each issuing all B-slice loads of a leaf at once:
+//   kb_proj   t_b = B_b' X[cols(b)] for every low-rank leaf, as partial sums
+//             per 256-column super-tile.  One workgroup per super-tile
+//             stages X[super-tile] in LDS as split BF16 (hi + lo); the rank
+//             columns crossing the super-tile (several leaves, ranks 1-2
+//             each here) form groups of 16, the N side of
 //             v_mfma_f32_16x16x32_bf16 with X' as the A operand (M =
-//             right-hand sides) and the B slice as the B operand (N = rank).
+//             right-hand sides).  Each lane loads its column's B values for
+//             all K-steps in one round trip, overlapped with the X stage;
 //             X is read from HBM once per product, not once per leaf.
 //   kb_reduce sums the per-super-tile partials of leaves wider than one
-//             super-tile, in a fixed order (deterministic, no atomics).
-//   kb_rows   row owner: one workgroup per tile of <= 64 rows of Y, one
-//             wavefront per 16-row M-tile.  A tile is a list of K segments:
-//             each dense leaf crossing it (A = its rows, B = X rows of its
-//             columns) and one panel holding the A columns of every low-rank
-//             leaf crossing it (B = the matching t vectors).  All A operands
-//             are pre-packed in MFMA fragment order (16 rows x 8 k per lane
-//             group; zero rows and octets are not stored).  Per batch of
-//             K-steps each wave issues all its A-fragment loads while the
-//             workgroup stages the B rows (X windows, t vectors) into LDS
-//             with coalesced loads: one memory round trip per batch.  Y
-//             leaves through LDS as contiguous column segments, each element
-//             written once.  Workgroups are remapped so consecutive tiles run
-//             on the same XCD and share X rows in its L2.
+//             super-tile, in a fixed order (deterministic, no atomics),
+//             largest leaves first.
+//   kb_strips row owner: one wavefront per 16-row strip, four per
+//             workgroup.  A strip is a list of K-steps of 32 columns: each
+//             dense leaf crossing it (A = its rows, B = X rows of its
+//             columns) and a panel holding the A columns of every low-rank
+//             leaf crossing it (B = the matching t rows).  A operands are
+//             pre-packed in MFMA fragment order (zero rows and octets not
+//             stored) and loaded straight into registers, K-steps in
+//             batches; X rows come from a workgroup window staged in LDS.
+//             Each element of Y is written once.  Workgroups are remapped
+//             so consecutive strips run on the same XCD and share X rows in
+//             its L2.
 // Storage modes: HM_BF16X2 (dense leaves as hi+lo BF16 planes, 3 MFMAs per
 // product term: hi*hi + hi*lo + lo*hi; low-rank factors FP32), HM_F32 (dense
 // leaves FP32, split in registers, same 3 MFMAs), HM_BF16 (dense leaves and
@@ -106,153 +107,203 @@ __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
 // projection
 constexpr int XOCT = PROJ_SUPER / 8;     // row octets of a super-tile
 constexpr int XO_LD = XOCT + 1;          // [rhs][octet] stride: conflict-free reads and writes
+constexpr int PKS = PROJ_SUPER / 32;     // K-steps of a super-tile
 
+// B operand of the K-steps of one 16-column group, this lane's column
+// (descriptor cd) and row octet g: 8 values per K-step, zero outside the
+// column's K-step range [s0, s1).  Raw values; split by split_v.
 template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_proj(const PTile *__restrict__ ptiles,
-                                               const PItem *__restrict__ pitems,
+struct VRaw {
+    static constexpr int W = MODE == MODE_F32 ? 2 : (MODE == MODE_BF16X2 ? 2 : 1);   // 16-byte words per K-step
+    uint4 w[PKS][W];
+};
+
+// The slice arena starts with VZERO zero elements: a lane whose column does
+// not reach K-step s loads those instead, so every load is the arena base
+// plus a 32-bit element offset (one address register, no branch).
+template <int MODE>
+__device__ __forceinline__ void load_v(const void *vslc, const uint4 &cd, int g, VRaw<MODE> &v) {
+    const int s0 = cd.y & 255, s1 = (cd.y >> 8) & 255;
+    const uint32_t base = cd.x * 8u + 8u * static_cast<uint32_t>(g) - 32u * static_cast<uint32_t>(s0);
+    const uint32_t len = static_cast<uint32_t>(s1 - s0) * 32u;
+#pragma unroll
+    for (int s = 0; s < PKS; ++s) {
+        const bool on = s >= s0 && s < s1;
+        const uint32_t e = on ? base + 32u * s : 0u;
+        if (MODE == MODE_F32) {
+            const uint4 *p = reinterpret_cast<const uint4 *>(static_cast<const float *>(vslc) + e);
+            v.w[s][0] = p[0];
+            v.w[s][1] = p[1];
+        } else {
+            const uint16_t *vp = static_cast<const uint16_t *>(vslc);
+            v.w[s][0] = *reinterpret_cast<const uint4 *>(vp + e);
+            if (MODE == MODE_BF16X2)
+                v.w[s][1] = *reinterpret_cast<const uint4 *>(vp + (on ? e + len : 0u));
+        }
+    }
+}
+
+// Projection: t_b = B_b' X[cols(b)] for every low-rank leaf, as partial sums
+// per super-tile of PROJ_SUPER columns of X.  One workgroup per super-tile:
+// it stages X[super-tile] in LDS as split BF16 (hi + lo); the rank columns
+// (leaf, column) crossing the super-tile form groups of 16, the N side of
+// v_mfma_f32_16x16x32_bf16 (A operand X', M = right-hand sides), and
+// wavefront w takes groups w, w + 4, ...  Each lane loads its column's B
+// values for every K-step at once, in the same round trip as the X stage
+// (descriptors and X first, then B).  Column descriptors (16 per group,
+// gps groups per super-tile): x = B offset / 8, y = s0 | s1 << 8 |
+// ngroups << 16, z = float offset of the result row t[j][0..NB) (-1: no
+// column).
+template <int MODE>
+__global__ __launch_bounds__(256) void kb_proj(int64_t gps, const uint4 *__restrict__ pcol,
                                                const void *__restrict__ vslc,
                                                const float *__restrict__ zeros,
                                                const float *__restrict__ X, int64_t ldx,
                                                int64_t ncols, int ncol, float *__restrict__ out) {
     __shared__ uint4 xs[2][NB][XO_LD];              // [hi/lo][rhs][row octet], 8 BF16 each
-    const PTile P = ptiles[blockIdx.x];
-    // Stage X[col0:col0+rows] as split BF16 in (column, 4-row quad) units: a
-    // wavefront reads 1 KB of one column per instruction; all loads first.
-    {
-        constexpr int XQ = PROJ_SUPER / 4;
-        constexpr int XIT = NB * XQ / 256;
-        f4u xv[XIT];
-#pragma unroll
-        for (int u = 0; u < XIT; ++u) {
-            const int p = threadIdx.x + 256 * u;
-            const int q = p % XQ;
-            const int rhs = p / XQ;
-            const bool full = rhs < ncol && 4 * q + 4 <= P.rows;
-            xv[u] = *reinterpret_cast<const f4u *>(full ? X + static_cast<int64_t>(rhs) * ldx + P.col0 + 4 * q : zeros);
-        }
-        // the quad cut by the matrix end: element loads after the vector ones
-#pragma unroll
-        for (int u = 0; u < XIT; ++u) {
-            const int p = threadIdx.x + 256 * u;
-            const int q = p % XQ;
-            const int rhs = p / XQ;
-            if (rhs < ncol && 4 * q < P.rows && 4 * q + 4 > P.rows) {
-                const float *src = X + static_cast<int64_t>(rhs) * ldx + P.col0 + 4 * q;
-                f4u v = {0.f, 0.f, 0.f, 0.f};
-                for (int e = 0; 4 * q + e < P.rows; ++e) v[e] = src[e];
-                xv[u] = v;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < XIT; ++u) {
-            const int p = threadIdx.x + 256 * u;
-            const int q = p % XQ;
-            const int rhs = p / XQ;
-            uint2 hi, lo;
-            split2(xv[u][0], xv[u][1], hi.x, lo.x);
-            split2(xv[u][2], xv[u][3], hi.y, lo.y);
-            *(reinterpret_cast<uint2 *>(&xs[0][rhs][q >> 1]) + (q & 1)) = hi;
-            *(reinterpret_cast<uint2 *>(&xs[1][rhs][q >> 1]) + (q & 1)) = lo;
-        }
-    }
-    __syncthreads();
-    const int wave = threadIdx.x >> 6;
+    const int64_t col0 = static_cast<int64_t>(blockIdx.x) * PROJ_SUPER;
+    const int rows = static_cast<int>(min<int64_t>(PROJ_SUPER, ncols - col0));
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int l = threadIdx.x & 63;
     const int r16 = l & 15;
     const int g = l >> 4;
+    const uint4 *gdesc = pcol + static_cast<int64_t>(blockIdx.x) * gps * 16;
+    // every wavefront loads a descriptor (the last group's when it has none)
+    // and empties its K-step range when the index is past the groups
+    uint4 cd = gdesc[min(wave, static_cast<int>(gps) - 1) * 16 + r16];
+    if (wave >= gps) cd = uint4{0u, cd.y & 0xffff0000u, 0xffffffffu, 0u};
+    // Stage X[col0:col0+rows] in (column, 4-row quad) units: thread t reads
+    // quad t % 64 of columns t / 64 + 4u, a wavefront 1 KB of one column per
+    // instruction; all loads first.
+    constexpr int XQ = PROJ_SUPER / 4;
+    static_assert(XQ == 64, "one quad per lane");
+    constexpr int XIT = NB * XQ / 256;
+    const int q = threadIdx.x & 63;
+    const int rhs0 = threadIdx.x >> 6;
+    const float *xb = X + col0 + 4 * q + static_cast<int64_t>(rhs0) * ldx;
+    const int64_t xstride = 4 * ldx;
+    const bool qfull = 4 * q + 4 <= rows;
+    f4u xv[XIT];
+#pragma unroll
+    for (int u = 0; u < XIT; ++u) {
+        const float *src = xb + u * xstride;
+        const bool full = qfull && rhs0 + 4 * u < ncol;
+        xv[u] = *reinterpret_cast<const f4u *>(full ? src : zeros);
+    }
+    if (rows < PROJ_SUPER) {      // last super-tile: the quad cut by the matrix end
+#pragma unroll
+        for (int u = 0; u < XIT; ++u) {
+            if (rhs0 + 4 * u < ncol && 4 * q < rows && !qfull) {
+                const float *src = xb + u * xstride;
+                f4u v = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int e = 0; e < 3; ++e)
+                    if (4 * q + e < rows) v[e] = src[e];
+                xv[u] = v;
+            }
+        }
+    }
+    // B values of the wavefront's first group (addresses from its descriptors)
+    VRaw<MODE> vr;
+    load_v<MODE>(vslc, cd, g, vr);
+#pragma unroll
+    for (int u = 0; u < XIT; ++u) {
+        const int rhs = rhs0 + 4 * u;
+        uint2 hi, lo;
+        split2(xv[u][0], xv[u][1], hi.x, lo.x);
+        split2(xv[u][2], xv[u][3], hi.y, lo.y);
+        *(reinterpret_cast<uint2 *>(&xs[0][rhs][q >> 1]) + (q & 1)) = hi;
+        *(reinterpret_cast<uint2 *>(&xs[1][rhs][q >> 1]) + (q & 1)) = lo;
+    }
+    __syncthreads();
+    const int ngroups = static_cast<int>(__builtin_amdgcn_readfirstlane(cd.y >> 16));
     const bf16x8 zero8 = __builtin_bit_cast(bf16x8, short8{0, 0, 0, 0, 0, 0, 0, 0});
-    constexpr int KS = PROJ_SUPER / 32 + 1;   // K-steps an item can span
-    for (int q = P.ib + wave; q < P.ie; q += 4) {
-        const PItem it = pitems[q];
-        const int nK = (it.nrow + 31) >> 5;
-        const bool kon = r16 < it.k;
-        // every B-slice load of the item is issued before the first MFMA;
-        // lanes without data read the zero buffer
-        bf16x8 vh[KS], vl[KS];
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int rloc = 32 * s + 8 * g;
-            const bool on = s < nK && kon && rloc < it.nrow;
-            const int64_t e0 = it.voff + static_cast<int64_t>(r16) * it.nrow + rloc;
-            if (MODE == MODE_F32) {
-                const float *vp = on ? static_cast<const float *>(vslc) + e0 : zeros;
-                const float4 a = reinterpret_cast<const float4 *>(vp)[0];
-                const float4 b = reinterpret_cast<const float4 *>(vp)[1];
-                float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-                split8hw(v, vh[s], vl[s]);
-            } else {
-                const uint16_t *vp = static_cast<const uint16_t *>(vslc);
-                const uint16_t *zb = reinterpret_cast<const uint16_t *>(zeros);
-                vh[s] = load_bf8(on ? vp + e0 : zb);
-                vl[s] = MODE == MODE_BF16X2 ? load_bf8(on ? vp + e0 + static_cast<int64_t>(it.k) * it.nrow : zb)
-                                            : zero8;
-            }
-        }
+    // one group: MFMAs over the K-steps, then the 32 x 16 result rows
+    auto group = [&]() {
         floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-        // MFMAs run for the whole wavefront (wave-uniform condition); lanes
-        // past the slice hold zero operands
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            if (s < nK) {
-                bf16x8 xh0 = zero8, xl0 = zero8, xh1 = zero8, xl1 = zero8;
-                if (32 * s + 8 * g < it.nrow) {
-                    const int oct = (it.ls >> 3) + 4 * s + g;
-                    xh0 = __builtin_bit_cast(bf16x8, xs[0][r16][oct]);
-                    xl0 = __builtin_bit_cast(bf16x8, xs[1][r16][oct]);
-                    xh1 = __builtin_bit_cast(bf16x8, xs[0][16 + r16][oct]);
-                    xl1 = __builtin_bit_cast(bf16x8, xs[1][16 + r16][oct]);
-                }
-                acc0 = mfma(xh0, vh[s], acc0);
-                acc0 = mfma(xl0, vh[s], acc0);
-                acc1 = mfma(xh1, vh[s], acc1);
-                acc1 = mfma(xl1, vh[s], acc1);
-                if (MODE != MODE_BF16) {
-                    acc0 = mfma(xh0, vl[s], acc0);
-                    acc1 = mfma(xh1, vl[s], acc1);
-                }
+        for (int s = 0; s < PKS; ++s) {
+            bf16x8 vh, vl;
+            if (MODE == MODE_F32) {
+                const float v[8] = {__uint_as_float(vr.w[s][0].x), __uint_as_float(vr.w[s][0].y),
+                                    __uint_as_float(vr.w[s][0].z), __uint_as_float(vr.w[s][0].w),
+                                    __uint_as_float(vr.w[s][1].x), __uint_as_float(vr.w[s][1].y),
+                                    __uint_as_float(vr.w[s][1].z), __uint_as_float(vr.w[s][1].w)};
+                split8hw(v, vh, vl);
+            } else {
+                vh = __builtin_bit_cast(bf16x8, vr.w[s][0]);
+                vl = MODE == MODE_BF16X2 ? __builtin_bit_cast(bf16x8, vr.w[s][VRaw<MODE>::W - 1]) : zero8;
+            }
+            const int oct = 4 * s + g;
+            const bf16x8 xh0 = __builtin_bit_cast(bf16x8, xs[0][r16][oct]);
+            const bf16x8 xl0 = __builtin_bit_cast(bf16x8, xs[1][r16][oct]);
+            const bf16x8 xh1 = __builtin_bit_cast(bf16x8, xs[0][16 + r16][oct]);
+            const bf16x8 xl1 = __builtin_bit_cast(bf16x8, xs[1][16 + r16][oct]);
+            acc0 = mfma(xh0, vh, acc0);
+            acc0 = mfma(xl0, vh, acc0);
+            acc1 = mfma(xh1, vh, acc1);
+            acc1 = mfma(xl1, vh, acc1);
+            if (MODE != MODE_BF16) {
+                acc0 = mfma(xh0, vl, acc0);
+                acc1 = mfma(xh1, vl, acc1);
             }
         }
-        // D[i = rhs][j = rank]: lane holds rhs 4g..4g+3 (+16 for acc1), rank r16
-        if (kon) {
-            float *dst = out + it.out + static_cast<int64_t>(r16) * NB + 4 * g;
+        // D[i = rhs][j = column]: lane holds rhs 4g..4g+3 (+16 for acc1) of column r16
+        const int o = static_cast<int>(cd.z);
+        if (o >= 0) {
+            float *dst = out + o + 4 * g;
             reinterpret_cast<float4 *>(dst)[0] = make_float4(acc0[0], acc0[1], acc0[2], acc0[3]);
             reinterpret_cast<float4 *>(dst + 16)[0] = make_float4(acc1[0], acc1[1], acc1[2], acc1[3]);
         }
+    };
+    // The first group runs unconditionally (a wavefront without one has an
+    // empty descriptor: zero operands, no store), so its B loads stay ahead
+    // of the barrier; later groups load descriptors, then B values.
+    group();
+    for (int gi = wave + 4; gi < ngroups; gi += 4) {
+        cd = gdesc[gi * 16 + r16];
+        load_v<MODE>(vslc, cd, g, vr);
+        group();
     }
 }
 
 // ---------------------------------------------------------------------------
 // t = sum of the per-super-tile partials of one leaf.  Threads split into
-// G = 256/len groups; group q sums chunks q, q+G, ... (eight loads in flight)
-// and the groups are combined in LDS in ascending order: a fixed order, so
-// the result is deterministic.
+// G = 256/len groups; group q sums chunks q, q+G, ... in ascending order
+// (RED_BATCH loads in flight per thread) and the groups are combined in LDS
+// in ascending order: a fixed order, so the result is deterministic.  Items
+// come largest first.
+constexpr int RED_BATCH = 32;
 __global__ __launch_bounds__(256) void kb_reduce(const BRed *__restrict__ red,
                                                  const float *__restrict__ part,
+                                                 const float *__restrict__ zeros,
                                                  float *__restrict__ t) {
     __shared__ float sh[256];
     const BRed r = red[blockIdx.x];
-    const int G = r.len >= 256 ? 1 : 256 / r.len;
-    for (int e0 = 0; e0 < r.len; e0 += 256 / G) {
-        const int e = e0 + static_cast<int>(threadIdx.x) % (256 / G);
-        const int q = static_cast<int>(threadIdx.x) / (256 / G);
+    const int per = r.len >= 256 ? 256 : r.len;    // elements per pass
+    const int G = 256 / per;
+    for (int e0 = 0; e0 < r.len; e0 += per) {
+        const int e = e0 + static_cast<int>(threadIdx.x) % per;
+        const int q = static_cast<int>(threadIdx.x) / per;
         float acc = 0.0f;
         if (e < r.len && q < G) {
             const float *src = part + r.pOff + e;
-            int c = q;
-            for (; c + 7 * G < r.nchunk; c += 8 * G) {
-                float v[8];
+            for (int c0 = q; c0 < r.nchunk; c0 += RED_BATCH * G) {
+                float v[RED_BATCH];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = src[static_cast<int64_t>(c + u * G) * r.len];
+                for (int u = 0; u < RED_BATCH; ++u) {
+                    const int c = c0 + u * G;
+                    v[u] = *(c < r.nchunk ? src + static_cast<int64_t>(c) * r.len : zeros);
+                }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) acc += v[u];
+                for (int u = 0; u < RED_BATCH; ++u) acc += v[u];
             }
-            for (; c < r.nchunk; c += G) acc += src[static_cast<int64_t>(c) * r.len];
         }
         sh[threadIdx.x] = acc;
         __syncthreads();
         if (q == 0 && e < r.len) {
             float s = acc;
-            for (int g = 1; g < G; ++g) s += sh[g * (256 / G) + threadIdx.x];
+            for (int gq = 1; gq < G; ++gq) s += sh[gq * per + threadIdx.x];
             t[r.tOff + e] = s;
         }
         __syncthreads();
@@ -539,17 +590,17 @@ static hipError_t run_b(const Direction &d, const void *zeros, int ncol, const f
                         hipEvent_t *ev) {
     const Batched &b = d.bat;
     hipError_t e = hipSuccess;
-    if (b.nptiles > 0) {
-        hipLaunchKernelGGL(kb_proj<MODE>, dim3(static_cast<unsigned>(b.nptiles)), dim3(256), 0, s,
-                           b.ptiles, b.pitems, b.vslc, static_cast<const float *>(zeros), X, ldx,
+    if (b.nst > 0 && b.gps > 0) {
+        hipLaunchKernelGGL(kb_proj<MODE>, dim3(static_cast<unsigned>(b.nst)), dim3(256), 0, s,
+                           b.gps, b.pcol, b.vslc, static_cast<const float *>(zeros), X, ldx,
                            d.ncols, ncol, b.t);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (b.nptiles > 0 && b.nred > 0) {
+    if (b.nst > 0 && b.gps > 0 && b.nred > 0) {
         hipLaunchKernelGGL(kb_reduce, dim3(static_cast<unsigned>(b.nred)), dim3(256), 0, s, b.red,
-                           static_cast<const float *>(b.t) + b.tLen, b.t);
+                           static_cast<const float *>(b.t) + b.tLen, static_cast<const float *>(zeros), b.t);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -116,32 +116,17 @@ constexpr int SREC_MAXK = (128 - SREC_HDR) / 4;
 // values (lane (g, r) reads values [8g, 8g + 8) of strip row roff + r).
 constexpr int SK_KIND_PANEL = 1;
 
-// Projection work unit: columns [col0, col0 + rows) of X and the B slices
-// of items [ib, ie) (bytes [vbeg, vbeg + vlen) of the slice arena, split
-// BF16 planes or FP32), all staged in LDS.  A super-tile whose slices exceed
-// the stage or PITEMS_MAX items is split into several units.
-constexpr int VSTAGE_BYTES = 32768;
-constexpr int PITEMS_MAX = 64;
-struct PTile {
-    int64_t col0;
-    int64_t vbeg;
-    int32_t vlen;
-    int32_t rows;
-    int32_t ib, ie;
-};
-
-// Projection item: partial^T = X[slice]^T V[slice] for one low-rank leaf.
-// The V slice covers super-tile rows [ls, ls + nrow) (ls and nrow multiples
-// of 8, zero rows where the leaf does not reach), column-major, ld nrow; for
-// split-BF16 storage the lo plane follows the hi plane (k*nrow elements).
-struct PItem {
-    int64_t voff;   // V slice base (elements)
-    int64_t out;    // float offset of the k x NB result ([j][NB])
-    int32_t ls;
-    int32_t nrow;
-    int32_t k;
-    int32_t pad;
-};
+// Projection: the rank columns (leaf b, column j) crossing a super-tile of
+// PROJ_SUPER columns of X form groups of 16; gps groups per super-tile (the
+// largest count of the plan; unused groups hold empty descriptors).  Column
+// descriptor (uint4): x = offset / 8 of the column's B values in the slice
+// arena, y = s0 | s1 << 8 | ngroups << 16 with [s0, s1) the 32-row K-steps
+// of the super-tile the leaf reaches (values stored for those rows only,
+// zeros around the leaf; split-BF16: hi rows then lo rows), z = float offset
+// of the result row t[j][0..NB) (partial or t; -1 for no column).  The
+// arena starts with VZERO zero elements, read for K-steps a column lacks;
+// element offsets stay below 2^32.
+constexpr int VZERO = 16;
 
 // Batched reduction: t[tOff + e] = sum_c part[pOff + c*len + e], e < len.
 struct BRed {
@@ -157,13 +142,12 @@ struct Batched {
     uint32_t *soff = nullptr;     // word offset of each strip record
     uint32_t *recs = nullptr;     // strip records
     int32_t *wgx = nullptr;       // per workgroup: X window (first row, rows; 0 rows: not staged)
-    PTile *ptiles = nullptr;
-    int64_t nptiles = 0;
-    PItem *pitems = nullptr;
+    uint4 *pcol = nullptr;        // projection column descriptors [nst][gps][16]
+    int64_t nst = 0, gps = 0;     // super-tiles, groups per super-tile
     BRed *red = nullptr;
     int64_t nred = 0;
     void *frag = nullptr;    // A fragments of every strip K-step
-    void *vslc = nullptr;    // V slices per super-tile
+    void *vslc = nullptr;    // B slices per super-tile column
     float *t = nullptr;      // [t (tLen) | partials (partLen) | NB zeros]
     int64_t tLen = 0, partLen = 0;
 };

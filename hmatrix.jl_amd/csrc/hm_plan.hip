@@ -198,8 +198,8 @@ struct BatchedHost {
     int64_t nstrips = 0;
     std::vector<uint32_t> soff, recs;
     std::vector<int32_t> wgx;
-    std::vector<hm::PTile> pt;
-    std::vector<hm::PItem> pitems;
+    std::vector<uint32_t> pcol;        // projection column descriptors, 4 words each
+    int64_t nst = 0, gps = 0;
     std::vector<hm::BRed> red;
     Arena fr, vs;
     int64_t tLen = 0, partLen = 0;
@@ -221,7 +221,7 @@ bool layout_batched(int mode, int64_t nrows, int64_t ncols, const std::vector<OB
             tLen += obs[b].k * NB;
         }
     // chunked projections: partial sums of leaves wider than one super-tile
-    std::vector<int64_t> pOff(obs.size(), -1), nch(obs.size(), 0), chunkIdx(obs.size(), 0);
+    std::vector<int64_t> pOff(obs.size(), -1), nch(obs.size(), 0);
     int64_t partLen = 0;
     std::vector<BRed> red;
     for (size_t b = 0; b < obs.size(); ++b) {
@@ -442,78 +442,74 @@ bool layout_batched(int mode, int64_t nrows, int64_t ncols, const std::vector<OB
             recs.insert(recs.end(), rec.begin(), rec.end());
         }
     }
-    // ---- projection: super-tiles of PROJ_SUPER columns
+    // ---- projection: super-tiles of PROJ_SUPER columns.  The rank columns
+    // (leaf, j) crossing a super-tile are cut into groups of 16; each column
+    // stores its B values over the 32-row K-steps [s0, s1) of the super-tile
+    // its leaf reaches (zeros around the leaf), hi rows then lo rows.
     const int64_t nst = (ncols + PROJ_SUPER - 1) / PROJ_SUPER;
     std::vector<std::vector<int32_t>> perst(static_cast<size_t>(nst));
+    std::vector<int64_t> ncolst(static_cast<size_t>(nst), 0);
     for (size_t b = 0; b < obs.size(); ++b) {
         const OB &o = obs[b];
         if (o.kind != KIND_RK || !contributes(o)) continue;
-        for (int64_t st = o.col0 / PROJ_SUPER; st <= (o.col0 + o.n - 1) / PROJ_SUPER; ++st)
+        for (int64_t st = o.col0 / PROJ_SUPER; st <= (o.col0 + o.n - 1) / PROJ_SUPER; ++st) {
             perst[st].push_back(static_cast<int32_t>(b));
+            ncolst[st] += o.k;
+        }
     }
+    int64_t gps = 0;
+    for (int64_t st = 0; st < nst; ++st) gps = std::max(gps, (ncolst[st] + 15) / 16);
+    if (tLen + partLen >= (int64_t(1) << 31)) throw std::length_error("t and partials exceed 2^31 floats");
+    std::vector<uint32_t> pcol(static_cast<size_t>(nst * gps * 16 * 4), 0u);
     Arena vs;
-    vs.mode = mode == 1 ? 1 : 2;      // split-BF16 slices: hi plane then lo plane
-    std::vector<PTile> pt;
-    std::vector<PItem> pitems;
+    vs.mode = mode == 1 ? 1 : 2;
+    for (int i = 0; i < VZERO; ++i) vs.push(0.0);
     for (int64_t st = 0; st < nst; ++st) {
-        if (perst[st].empty()) continue;
-        PTile P{};
-        P.col0 = st * PROJ_SUPER;
-        P.rows = static_cast<int32_t>(std::min<int64_t>(PROJ_SUPER, ncols - P.col0));
-        auto open_unit = [&]() {
-            vs.align(8);
-            P.ib = static_cast<int32_t>(pitems.size());
-            P.vbeg = vs.size();
-        };
-        auto close_unit = [&]() {
-            vs.align(8);
-            P.ie = static_cast<int32_t>(pitems.size());
-            P.vlen = static_cast<int32_t>(vs.size() - P.vbeg);
-            if (P.ie > P.ib) pt.push_back(P);
-        };
-        open_unit();
+        const int64_t ng = (ncolst[st] + 15) / 16;
+        uint32_t *dst = pcol.data() + st * gps * 64;
+        for (int64_t q = 0; q < gps * 16; ++q) {
+            dst[4 * q + 1] = static_cast<uint32_t>(ng << 16);
+            dst[4 * q + 2] = 0xffffffffu;
+        }
+        int64_t q = 0;
         for (int32_t b : perst[st]) {
             const OB &o = obs[b];
-            const int64_t a = std::max(o.col0, P.col0) - P.col0;
-            const int64_t e = std::min(o.col0 + o.n, P.col0 + P.rows) - P.col0;
-            const int64_t ls = a & ~int64_t(7);
-            const int64_t nrow = ((e + 7) & ~int64_t(7)) - ls;
-            const int64_t esz = mode == 1 ? 4 : 2;
-            const int64_t ibytes = nrow * o.k * esz * (mode == 3 ? 2 : 1);
-            if (ibytes > VSTAGE_BYTES) throw std::length_error("rank too large for the projection stage");
-            vs.align(8);
-            if ((vs.size() - P.vbeg) * esz + ibytes > VSTAGE_BYTES || pitems.size() - P.ib >= size_t(PITEMS_MAX)) {
-                close_unit();
-                open_unit();
-            }
-            PItem it{};
-            it.voff = vs.size();
-            it.ls = static_cast<int32_t>(ls);
-            it.nrow = static_cast<int32_t>(nrow);
-            it.k = static_cast<int32_t>(o.k);
-            if (nch[b] > 1) it.out = tLen + pOff[b] + (chunkIdx[b]++) * o.k * NB;
-            else it.out = tOff[b];
+            const int64_t a = std::max(o.col0, st * PROJ_SUPER) - st * PROJ_SUPER;
+            const int64_t e = std::min(o.col0 + o.n, (st + 1) * PROJ_SUPER) - st * PROJ_SUPER;
+            const int64_t s0 = a / 32, s1 = (e + 31) / 32;
+            const int64_t chunk = st - o.col0 / PROJ_SUPER;
             const double *Bf = fac + o.offB;
-            for (int plane = 0; plane < (mode == 3 ? 2 : 1); ++plane)
-                for (int64_t j = 0; j < o.k; ++j)
-                    for (int64_t r = 0; r < nrow; ++r) {
-                        const int64_t i = P.col0 + ls + r - o.col0;
+            for (int64_t j = 0; j < o.k; ++j, ++q) {
+                vs.align(8);
+                const int64_t base = vs.size();
+                // element offsets (both planes) stay below 2^32
+                if (base + 2 * 32 * (s1 - s0) >= (int64_t(1) << 32)) return false;
+                for (int plane = 0; plane < (mode == 3 ? 2 : 1); ++plane)
+                    for (int64_t r = 32 * s0; r < 32 * s1; ++r) {
+                        const int64_t i = st * PROJ_SUPER + r - o.col0;
                         const double v = (i >= 0 && i < o.n) ? Bf[i + j * o.n] : 0.0;
                         if (plane == 0) vs.push(v);
                         else vs.push_lo(v);
                     }
-            pitems.push_back(it);
+                const int64_t out = nch[b] > 1 ? tLen + pOff[b] + chunk * o.k * NB + j * NB : tOff[b] + j * NB;
+                dst[4 * q + 0] = static_cast<uint32_t>(base / 8);
+                dst[4 * q + 1] = static_cast<uint32_t>(s0 | (s1 << 8) | (ng << 16));
+                dst[4 * q + 2] = static_cast<uint32_t>(out);
+            }
         }
-        close_unit();
     }
-    if (pitems.size() >= (size_t(1) << 31)) throw std::length_error("work lists exceed 2^31 entries");
+    // reductions, largest first (they start first)
+    std::stable_sort(red.begin(), red.end(), [](const BRed &x, const BRed &y) {
+        return static_cast<int64_t>(x.nchunk) * x.len > static_cast<int64_t>(y.nchunk) * y.len;
+    });
     recs.resize(recs.size() + 128, 0u);   // the row pass reads 128 words from every record start
     H.nstrips = nstrips;
     H.wgx = std::move(wgx);
     H.soff = std::move(soff);
     H.recs = std::move(recs);
-    H.pt = std::move(pt);
-    H.pitems = std::move(pitems);
+    H.pcol = std::move(pcol);
+    H.nst = gps > 0 ? nst : 0;
+    H.gps = gps;
     H.red = std::move(red);
     H.fr = std::move(fr);
     H.vs = std::move(vs);
@@ -532,9 +528,9 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
     B.soff = dev_upload(p, H.soff);
     B.recs = dev_upload(p, H.recs);
     B.wgx = dev_upload(p, H.wgx);
-    B.ptiles = dev_upload(p, H.pt);
-    B.nptiles = static_cast<int64_t>(H.pt.size());
-    B.pitems = dev_upload(p, H.pitems);
+    B.pcol = H.pcol.empty() ? nullptr : reinterpret_cast<uint4 *>(dev_upload(p, H.pcol));
+    B.nst = H.nst;
+    B.gps = H.gps;
     B.red = dev_upload(p, H.red);
     B.nred = static_cast<int64_t>(H.red.size());
     B.frag = H.fr.mode == 1 ? upload_vec(p, H.fr.f) : upload_vec(p, H.fr.h);
