@@ -144,8 +144,8 @@ struct Batched {
     int32_t *wgx = nullptr;       // per workgroup: X window (first row, rows; 0 rows: not staged)
     uint4 *pcol = nullptr;        // projection column descriptors [nst][gps][16]
     int64_t nst = 0, gps = 0;     // super-tiles, groups per super-tile
-    BRed *red = nullptr;
-    int64_t nred = 0;
+    BRed *red = nullptr;          // largest first
+    int64_t nred = 0, nredbig = 0; // items; leading items with more than 32 chunks
     void *frag = nullptr;    // A fragments of every strip K-step
     void *vslc = nullptr;    // B slices per super-tile column
     float *t = nullptr;      // [t (tLen) | partials (partLen) | NB zeros]

@@ -499,8 +499,10 @@ bool layout_batched(int mode, int64_t nrows, int64_t ncols, const std::vector<OB
         }
     }
     // reductions, largest first (they start first)
+    // (the items with more than 32 chunks, which the reduction gives a whole
+    // workgroup, then lead the list)
     std::stable_sort(red.begin(), red.end(), [](const BRed &x, const BRed &y) {
-        return static_cast<int64_t>(x.nchunk) * x.len > static_cast<int64_t>(y.nchunk) * y.len;
+        return x.nchunk != y.nchunk ? x.nchunk > y.nchunk : x.len > y.len;
     });
     recs.resize(recs.size() + 128, 0u);   // the row pass reads 128 words from every record start
     H.nstrips = nstrips;
@@ -533,6 +535,8 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
     B.gps = H.gps;
     B.red = dev_upload(p, H.red);
     B.nred = static_cast<int64_t>(H.red.size());
+    B.nredbig = 0;
+    while (B.nredbig < B.nred && H.red[static_cast<size_t>(B.nredbig)].nchunk > 32) ++B.nredbig;
     B.frag = H.fr.mode == 1 ? upload_vec(p, H.fr.f) : upload_vec(p, H.fr.h);
     B.vslc = H.vs.mode == 1 ? upload_vec(p, H.vs.f) : upload_vec(p, H.vs.h);
     B.tLen = H.tLen;
