@@ -6,7 +6,9 @@ admissibility (Adj = identity), ACA construction on the host in FP64;
 X = 32 standard-normal right-hand sides (seed 4).
 
 One step = one product Y = H*X of the 32 right-hand sides, inputs resident
-in HBM.  The batched path streams the factors once per step in the chosen
+in HBM.  Steps are independent products; by default two are in flight at a
+time on two default-priority HIP streams (--streams), which is the
+throughput reported; the one-at-a-time latency is reported beside it.  The batched path streams the factors once per step in the chosen
 storage (default: split BF16, FP32 accumulation).  The plain FP64 path
 (one matvec per right-hand side, the reference's H*X semantics) is timed
 beside it.  Several plan replicas are rotated so the working set exceeds the
@@ -108,6 +110,30 @@ def timed(torch, fn, steps: int, warmup: int, stream, after_warmup=None):
     return e0.elapsed_time(e1) / steps   # ms per step
 
 
+def timed_streams(torch, fn, steps: int, warmup: int, streams):
+    """Like timed(), with step i issued on streams[i % len(streams)]: the
+    steps are independent products, so with several streams consecutive
+    products overlap on the GPU.  Bracketed by synchronizations; the clock
+    runs on streams[0], which joins the others before the end event."""
+    for i in range(warmup):
+        fn(i, streams[i % len(streams)])
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(streams[0])
+    for s in streams[1:]:
+        s.wait_event(e0)
+    for i in range(steps):
+        fn(i, streams[i % len(streams)])
+    for s in streams[1:]:
+        ev = torch.cuda.Event()
+        ev.record(s)
+        streams[0].wait_event(ev)
+    e1.record(streams[0])
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps   # ms per step
+
+
 def cpu_baseline(f, seconds: float):
     """Oracle FP64 matvec, 1 thread, on the same factors; bounded sample."""
     from oracle import oracle
@@ -154,6 +180,57 @@ def c4_report(hm, torch, stream) -> None:
     d.close()
 
 
+def extras_report(hm, torch, stream, plan_b, Xb, Yb) -> dict:
+    """Timings of the other product forms (stderr and the JSON line's
+    "extras"): the transposed batched product H'X on the same plan replicas
+    (rotated); the plain
+    FP64 H*y at N=65536 (sorted uniform on [0,100], seed 1, Cauchy kernel,
+    MaxBlock 1024, MinBlock 64, MaxRank 16); the FP64 adjoint of
+    L = 0.5||Hx - b||^2 at N=65536 (seed 7, same kernel and parameters)."""
+    import scipy.sparse as sp
+    out = {}
+    R = len(plan_b)
+    ms = timed(torch, lambda i: plan_b[i % R].matmat(Xb[i % R], Yb[i % R], stream=stream, transpose=True),
+               20, 3, stream)
+    out["batched_transpose_us"] = ms * 1e3
+    log(f"H'X (batched, same plans, one at a time): {ms * 1e3:.1f} us per product of {Xb[0].shape[1]} rhs")
+
+    def cauchy_plan(seed):
+        par = hm.Params()
+        par.Geom = np.sort(np.random.default_rng(seed).uniform(0.0, 100.0, 65536))
+        par.Kernel = lambda x, y: 1.0 / (1.0 + (x - y) ** 2)
+        par.MaxBlock, par.MinBlock, par.MaxRank = 1024, 64, 16
+        par.ConsMethod = "aca2"
+        par.Adj = sp.identity(65536, format="csr")
+        c, H = hm.NewHmat(par)
+        return hm.flatten(H)
+
+    f2 = cauchy_plan(1)
+    d2 = hm.DeviceHmat(f2, "f64", {"no_batched": True})
+    x = torch.from_numpy(np.random.default_rng(2).standard_normal(65536)).to(d2.device)
+    y = torch.zeros(65536, dtype=torch.float64, device=d2.device)
+    ms = timed(torch, lambda i: d2.matvec(x, y, stream=stream), 50, 5, stream)
+    b = f2.blocks
+    dense = b[:, 0] == 0
+    nbytes = 8 * int((b[dense, 3] * b[dense, 4]).sum() + ((b[~dense, 3] + b[~dense, 4]) * b[~dense, 5]).sum()) \
+        + 16 * 65536
+    out["c2_fp64_matvec_us"] = ms * 1e3
+    out["c2_fp64_GBs"] = nbytes / (ms * 1e-3) / 1e9
+    log(f"C2 FP64 H*y (N=65536, {f2.nblocks} leaves): {ms * 1e3:.1f} us, {out['c2_fp64_GBs']:.0f} GB/s")
+    d2.close()
+
+    f5 = cauchy_plan(7)
+    d5 = hm.DeviceHmat(f5, "f64", {"no_batched": True})
+    rng = np.random.default_rng(8)
+    x5 = torch.from_numpy(rng.standard_normal(65536)).to(d5.device)
+    b5 = torch.from_numpy(rng.standard_normal(65536)).to(d5.device)
+    ms = timed(torch, lambda i: d5.adjoint(x5, b5, stream=stream), 20, 3, stream)
+    out["c5_adjoint_us"] = ms * 1e3
+    log(f"C5 FP64 adjoint (loss, residual, H' residual, factor gradients; N=65536): {ms * 1e3:.1f} us")
+    d5.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +240,9 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--storage", default="bf16x2", choices=["f32", "bf16", "bf16x2"])
     ap.add_argument("--replicas", type=int, default=4)
+    ap.add_argument("--streams", type=int, default=2,
+                    help="independent products in flight: step i runs on stream i %% S (default "
+                         "priority streams; replica r always on stream r %% S)")
     ap.add_argument("--plain-steps", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--sample-rows", type=int, default=4096)
@@ -171,6 +251,8 @@ def main() -> int:
                          "kernel the FETCH_SIZE/WRITE_SIZE counters are scaled against)")
     ap.add_argument("--c4", action="store_true",
                     help="also time the uneven 64-matrix batch (one launch) and report it on stderr")
+    ap.add_argument("--extras", action="store_true",
+                    help="also time H'X, the FP64 H*y at N=65536 and the FP64 adjoint at N=65536")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the product is "
@@ -185,6 +267,11 @@ def main() -> int:
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
+    S = max(1, args.streams)
+    if args.replicas % S:
+        print(f"--replicas ({args.replicas}) must be a multiple of --streams ({S})", file=sys.stderr)
+        return 2
+    streams = [stream] + [torch.cuda.Stream(device=dev) for _ in range(S - 1)]
 
     # batched low-precision path, rotated replicas
     t0 = time.time()
@@ -205,10 +292,14 @@ def main() -> int:
             p.timing(True)
             p.timing_read()
 
-    ms = timed(torch, step_batched, args.steps, args.warmup, stream)
-    # kernel durations: the same steps again, with HIP events recorded by the
-    # library on `stream` around each of the three kernels (the events add a
-    # few microseconds of gaps per step, so the step time above is taken
+    # throughput: S independent products in flight (replica r on stream r % S)
+    ms = timed_streams(torch, lambda i, st: plans[i % R].matmat(Xs[i % R], Ys[i % R], stream=st),
+                       args.steps, args.warmup, streams)
+    # one product at a time (latency of a product)
+    ms1 = timed(torch, step_batched, args.steps, args.warmup, stream)
+    # kernel durations: single-stream steps again, with HIP events recorded by
+    # the library on `stream` around each of the three kernels (the events add
+    # a few microseconds of gaps per step, so the step times above are taken
     # without them)
     ms_ev = timed(torch, step_batched, args.steps, 0, stream, start_kernel_timing)
     ph = {"proj_ms": 0.0, "reduce_ms": 0.0, "rows_ms": 0.0, "launches": 0}
@@ -219,8 +310,8 @@ def main() -> int:
             ph[k] += r[k]
     nl = max(ph["launches"], 1)
     kus = {k: ph[f"{k}_ms"] * 1e3 / nl for k in ("proj", "reduce", "rows")}
-    log(f"batched {args.storage}: {ms * 1e3:.1f} us per product of {nrhs} rhs ({ms_ev * 1e3:.1f} with "
-        f"kernel events); kernels (us/launch, "
+    log(f"batched {args.storage}: {ms * 1e3:.1f} us per product of {nrhs} rhs with {S} products in flight, "
+        f"{ms1 * 1e3:.1f} us one at a time ({ms_ev * 1e3:.1f} with kernel events); kernels (us/launch, "
         f"{ph['launches']} launches): proj {kus['proj']:.1f} reduce {kus['reduce']:.1f} rows {kus['rows']:.1f}")
 
     # accuracy of the batched result
@@ -231,20 +322,27 @@ def main() -> int:
     relerr_dense = float(np.linalg.norm(Yb[rows] - Yd) / np.linalg.norm(Yd))
     log(f"dense FP64 sample ({rows.size} rows) in {time.time() - t0:.1f}s; relerr {relerr_dense:.3e}")
 
-    # plain FP64 path: one matvec per right-hand side
-    p64 = hm.DeviceHmat(f, "f64")
+    # plain FP64 path: one matvec per right-hand side.  The same S-stream
+    # schedule: the 32 columns of a step are 32 independent FP64 matvecs,
+    # column j on stream j % S with its own plan (a plan's scratch serves one
+    # stream at a time)
+    p64s = [hm.DeviceHmat(f, "f64", {"no_batched": True}) for _ in range(S)]
+    p64 = p64s[0]
     X64 = p64.new_cols(N, nrhs)
     X64.copy_(torch.from_numpy(RHS).to(dev))
     Y64 = p64.new_cols(N, nrhs)
+    cols_x = [X64[:, j] for j in range(nrhs)]
+    cols_y = [Y64[:, j] for j in range(nrhs)]
 
-    def step_plain(i):
-        p64.matmat(X64, Y64, stream=stream)
+    def step_plain(i, st):
+        for j in range(nrhs):
+            p64s[j % S].matvec(cols_x[j], cols_y[j], stream=streams[j % S])
 
-    ms64 = timed(torch, step_plain, args.plain_steps, 1, stream)
+    ms64 = timed_streams(torch, step_plain, args.plain_steps, 1, streams)
     Yp = Y64.cpu().numpy()
     relerr_h64 = float(np.linalg.norm(Yb - Yp) / np.linalg.norm(Yp))
     relerr_dense64 = float(np.linalg.norm(Yp[rows] - Yd) / np.linalg.norm(Yd))
-    log(f"plain f64: {ms64 * 1e3:.1f} us per {nrhs} rhs; relerr vs dense {relerr_dense64:.3e}; "
+    log(f"plain f64 ({S} streams): {ms64 * 1e3:.1f} us per {nrhs} rhs; relerr vs dense {relerr_dense64:.3e}; "
         f"batched vs plain {relerr_h64:.3e}")
 
     # drop-in path with host buffers (PCIe-inclusive; never the reported value)
@@ -256,6 +354,7 @@ def main() -> int:
 
     if args.c4:
         c4_report(hm, torch, stream)
+    extras = extras_report(hm, torch, stream, plans, Xs, Ys) if args.extras else None
 
     if args.calibrate_copy_mb > 0:
         n = args.calibrate_copy_mb * (1 << 20) // 4
@@ -297,7 +396,7 @@ def main() -> int:
         "config": {"workload": f"H*X exp(-|x-y|) N={N} nrhs={nrhs} MaxRank=24 MinBlock=64 "
                                f"MaxBlock=N/4 weak-adm, {args.storage} storage",
                    "N": N, "nrhs": nrhs, "leaves": f.nblocks, "dense_leaves": f.counts()[0],
-                   "rk_leaves": f.counts()[1], "replicas": R},
+                   "rk_leaves": f.counts()[1], "replicas": R, "streams": S},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(args.storage, "kb_strips"),
@@ -309,6 +408,9 @@ def main() -> int:
                                   "traffic": profiled_traffic(args.storage, "kb_reduce")},
                     "kb_strips": {"us": kus["rows"], "algorithmic_bytes": kb["rows"], "GB/s": kgbs["rows"],
                                   "traffic": profiled_traffic(args.storage, "kb_strips")}},
+        "products_in_flight": S,
+        "single_stream_ms_per_step": ms1,
+        "single_stream_H-matvec/s": nrhs / (ms1 * 1e-3),
         "ms_per_step_with_kernel_events": ms_ev,
         "step_GBs": step_gbs,
         "step_frac": step_gbs / HBM_PEAK_GBS,
@@ -322,8 +424,10 @@ def main() -> int:
         "host_buffer_product_ms": pcie_s * 1e3,
         "algorithmic_bytes_per_step": bytes_b,
     }
+    if extras is not None:
+        line["extras"] = extras
     print(json.dumps(line), flush=True)
-    for p in plans + [p64]:
+    for p in plans + p64s:
         p.close()
     return 0
 
