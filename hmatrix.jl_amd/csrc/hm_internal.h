@@ -20,6 +20,8 @@ constexpr int PROJ_CHUNK = 1024;
 constexpr int PROJ_KMAX = 16;
 // Waves per workgroup in the generic kernels.
 constexpr int WAVES = 4;
+// Partial sums of the loss reduction (workgroups of its first pass, at most).
+constexpr int LOSS_PARTS = 256;
 // Size of the plan's zero buffer (covers one t vector and any 16-byte load).
 constexpr int ZERO_BYTES = 4096;
 
@@ -35,16 +37,18 @@ struct BlkDev {
     int32_t m, n, k, kind;
 };
 
-// Projection item: t[j] (+)= sum_{i in rows} B[i, j] x[col0 + i] for
-// j in [j0, j0 + kc).
+// Projection item: out[j] = sum_{i < rows} fac[base + j*ld + i] x[xoff + i]
+// for j < kc: rows [row0, row0 + rows) of rank columns [j0, j0 + kc) of one
+// leaf's B (col-major, ld = n), with base and xoff already pointing at
+// row row0 / column j0, so a wavefront needs this record only.
 struct ProjItem {
-    int32_t blk;
-    int32_t row0;     // first row of B
-    int32_t rows;
-    int32_t j0;
-    int32_t kc;
-    int32_t pad;
+    int64_t base;     // arena offset of B[row0, j0]
+    int64_t xoff;     // index into x of row row0 (col0 + row0)
     int64_t out;      // destination: partial buffer or t buffer
+    int32_t ld;
+    int32_t rows;
+    int32_t kc;       // 1 .. PROJ_KMAX
+    int32_t pad;
 };
 
 // Reduction item: t[tOff + j] = sum_c part[pOff + c*kc + j], with part the
@@ -173,14 +177,19 @@ struct Direction {
     Batched bat;
 };
 
-// Gradient item: elements [e0, e1) of one gradient block in the exported
-// arena layout.  kind 0: dC (m x n), 1: dA (m x k), 2: dB (n x k).
+// Gradient item: rows [i0, i0 + rows) x columns [j0, j0 + ncol) of one
+// gradient block in the exported arena layout (col-major, ld rows), one
+// wavefront each: lane = row within a 64-row strip, at most GRAD_STRIPS
+// strips.  kind 0: dC = res x' (m x n), 1: dA = res t' (m x k),
+// 2: dB = x s' (n x k).
+constexpr int GRAD_STRIPS = 16;
 struct GradItem {
-    int64_t dst;      // offset of the block in the gradient arena
-    int64_t vec0;     // first index into the vector of the block's rows (dC, dA: res; dB: x)
-    int64_t aux;      // dC: col0 (index into x); dA/dB: t offset
-    int64_t e0, e1;
+    int64_t dst;      // arena offset of element (i0, j0)
+    int64_t vec0;     // index of row i0 in the row vector (dC, dA: res; dB: x)
+    int64_t aux;      // index of column j0 in the column vector (dC: x; dA: t; dB: s)
     int32_t ld;       // rows of the block (m, or n for dB)
+    int32_t rows;     // <= 64 * GRAD_STRIPS
+    int32_t ncol;
     int32_t kind;
 };
 
@@ -216,7 +225,8 @@ struct Plan {
     GradItem *grad = nullptr;
     int64_t ngrad = 0;
     int32_t *rexp = nullptr;       // length fwd.tLen
-    void *lossbuf = nullptr;       // one double
+    void *lossbuf = nullptr;       // [loss | LOSS_PARTS partial sums], doubles
+    double *losshost = nullptr;    // pinned host copy of the loss
     int64_t ndata = 0;
     std::vector<void *> allocs;
     Timing timing;

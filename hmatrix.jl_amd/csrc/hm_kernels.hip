@@ -69,56 +69,87 @@ __device__ __forceinline__ double bcast(double v, int src) {
                                 static_cast<unsigned int>(lo));
 }
 
-// Pass 1: projection partials.  One wavefront per item; lanes stride the
-// rows of B, four 64-row strips in flight, each lane keeps up to PROJ_KMAX
-// rank accumulators, one wave reduction per rank column at the end.
+// Pass 1: projection partials.  One wavefront per item, compiled for the
+// item's exact rank-column count KC so that every load of a round is
+// unconditional and all of them issue before the first wait: a round covers
+// U 64-row strips of all KC columns (about 32 loads per lane), lanes stride
+// the rows, each lane keeps KC accumulators, one wave reduction per column.
+template <class St, int KC>
+__device__ __forceinline__ void proj_item(const ProjItem &w, const void *__restrict__ fac,
+                                          const void *__restrict__ fac_lo,
+                                          const typename St::acc *__restrict__ x,
+                                          typename St::acc *__restrict__ out, int lane) {
+    using A = typename St::acc;
+    constexpr int U = KC >= 32 ? 1 : (32 / KC > PROJ_CHUNK / 64 ? PROJ_CHUNK / 64 : 32 / KC);
+    const int rows = w.rows;
+    const int64_t ld = w.ld;
+    const A *xs = x + w.xoff;
+    A acc[KC];
+#pragma unroll
+    for (int j = 0; j < KC; ++j) acc[j] = A(0);
+    // Lanes past the item's rows load a clamped (valid) row and multiply by
+    // zero.
+    for (int r0 = 0; r0 < rows; r0 += 64 * U) {
+        A xv[U];
+        int rc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int r = r0 + 64 * u + lane;
+            rc[u] = min(r, rows - 1);
+            const A v = xs[rc[u]];
+            xv[u] = r < rows ? v : A(0);
+        }
+        A bv[KC][U];
+#pragma unroll
+        for (int j = 0; j < KC; ++j)
+#pragma unroll
+            for (int u = 0; u < U; ++u) bv[j][u] = St::ld(fac, fac_lo, w.base + j * ld + rc[u]);
+#pragma unroll
+        for (int j = 0; j < KC; ++j)
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[j] += bv[j][u] * xv[u];
+    }
+    // butterfly over all columns at once: KC independent exchanges per step
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int j = 0; j < KC; ++j) acc[j] += __shfl_xor(acc[j], o, 64);
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < KC; ++j) out[w.out + j] = acc[j];
+    }
+}
+
 template <class St>
-__global__ __launch_bounds__(256) void k_proj(const BlkDev *__restrict__ blk,
-                                              const ProjItem *__restrict__ items, int64_t nitems,
+__global__ __launch_bounds__(256) void k_proj(const ProjItem *__restrict__ items, int64_t nitems,
                                               const void *__restrict__ fac,
                                               const void *__restrict__ fac_lo,
                                               const typename St::acc *__restrict__ x,
                                               typename St::acc *__restrict__ out) {
-    using A = typename St::acc;
     const int lane = threadIdx.x & 63;
-    const int64_t it = static_cast<int64_t>(blockIdx.x) * WAVES + (threadIdx.x >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int64_t it = static_cast<int64_t>(blockIdx.x) * WAVES + wave;
     if (it >= nitems) return;
     const ProjItem w = items[it];
-    const BlkDev b = blk[w.blk];
-    const int64_t ld = b.n;
-    const int64_t base = b.offB + static_cast<int64_t>(w.j0) * ld;
-    const A *xs = x + b.col0;
-    A acc[PROJ_KMAX];
-#pragma unroll
-    for (int j = 0; j < PROJ_KMAX; ++j) acc[j] = A(0);
-    // Lanes past the item's rows load a clamped (valid) row and multiply by
-    // zero: every load is unconditional, so all of them issue before the
-    // first wait.
-    for (int r0 = 0; r0 < w.rows; r0 += 256) {
-        A xv[4];
-        int rc[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int r = r0 + 64 * u + lane;
-            rc[u] = min(r, w.rows - 1);
-            const A v = xs[w.row0 + rc[u]];
-            xv[u] = r < w.rows ? v : A(0);
-        }
-#pragma unroll
-        for (int j = 0; j < PROJ_KMAX; ++j) {
-            if (j < w.kc) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    acc[j] += St::ld(fac, fac_lo, base + w.row0 + rc[u] + j * ld) * xv[u];
-            }
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < PROJ_KMAX; ++j) {
-        if (j < w.kc) {
-            const A v = wave_sum(acc[j]);
-            if (lane == 0) out[w.out + j] = v;
-        }
+    static_assert(PROJ_KMAX == 16, "one case per rank-column count");
+    switch (w.kc) {
+        case 1: proj_item<St, 1>(w, fac, fac_lo, x, out, lane); break;
+        case 2: proj_item<St, 2>(w, fac, fac_lo, x, out, lane); break;
+        case 3: proj_item<St, 3>(w, fac, fac_lo, x, out, lane); break;
+        case 4: proj_item<St, 4>(w, fac, fac_lo, x, out, lane); break;
+        case 5: proj_item<St, 5>(w, fac, fac_lo, x, out, lane); break;
+        case 6: proj_item<St, 6>(w, fac, fac_lo, x, out, lane); break;
+        case 7: proj_item<St, 7>(w, fac, fac_lo, x, out, lane); break;
+        case 8: proj_item<St, 8>(w, fac, fac_lo, x, out, lane); break;
+        case 9: proj_item<St, 9>(w, fac, fac_lo, x, out, lane); break;
+        case 10: proj_item<St, 10>(w, fac, fac_lo, x, out, lane); break;
+        case 11: proj_item<St, 11>(w, fac, fac_lo, x, out, lane); break;
+        case 12: proj_item<St, 12>(w, fac, fac_lo, x, out, lane); break;
+        case 13: proj_item<St, 13>(w, fac, fac_lo, x, out, lane); break;
+        case 14: proj_item<St, 14>(w, fac, fac_lo, x, out, lane); break;
+        case 15: proj_item<St, 15>(w, fac, fac_lo, x, out, lane); break;
+        case 16: proj_item<St, 16>(w, fac, fac_lo, x, out, lane); break;
+        default: break;
     }
 }
 
@@ -204,66 +235,112 @@ __global__ __launch_bounds__(64 * NW) void k_rows(const TileDev *__restrict__ ti
 // Adjoint gradients of L = 0.5*||H x - b||^2 in the exported arena layout:
 // dC = res x', dA = res t' and dB = x s', with t = B'x and s = A'res taken
 // from the projection passes of the two products (scaled back by the
-// power-of-two balance of each rank column, exactly).  One workgroup per
-// item of <= GRAD_CHUNK contiguous gradient elements: every write is
-// coalesced.
+// power-of-two balance of each rank column, exactly).  One wavefront per
+// item: each lane loads its rows' vector values once, then walks the
+// item's columns, reading the column multiplier with a wave-uniform (scalar)
+// load and storing one 64-row strip per store instruction.  The loop issues
+// no vector load, so nothing waits for the stores in flight.
 template <class A>
-__global__ __launch_bounds__(256) void k_grad(const GradItem *__restrict__ items,
+__global__ __launch_bounds__(256) void k_grad(const GradItem *__restrict__ items, int64_t nitems,
                                               const int32_t *__restrict__ rexp,
                                               const A *__restrict__ x, const A *__restrict__ res,
                                               const A *__restrict__ t, const A *__restrict__ s,
                                               A *__restrict__ grad) {
-    const GradItem it = items[blockIdx.x];
-    for (int64_t e = it.e0 + threadIdx.x; e < it.e1; e += 256) {
-        const int64_t i = e % it.ld;
-        const int64_t j = e / it.ld;
-        A v;
-        if (it.kind == 0) {
-            v = res[it.vec0 + i] * x[it.aux + j];
-        } else if (it.kind == 1) {
-            const int64_t q = it.aux + j;
-            v = res[it.vec0 + i] * static_cast<A>(ldexp(static_cast<double>(t[q]), rexp[q]));
-        } else {
-            const int64_t q = it.aux + j;
-            v = x[it.vec0 + i] * static_cast<A>(ldexp(static_cast<double>(s[q]), -rexp[q]));
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * WAVES + wave;
+    if (id >= nitems) return;
+    const GradItem it = items[id];
+    const A *uv = (it.kind == 2 ? x : res) + it.vec0;
+    const A *wv = (it.kind == 0 ? x : (it.kind == 1 ? t : s)) + it.aux;
+    const int32_t *re = rexp + it.aux;
+    const int sg = it.kind == 2 ? -1 : 1;
+    const int rows = it.rows;
+    const int64_t ld = it.ld;
+    A u[GRAD_STRIPS];
+#pragma unroll
+    for (int q = 0; q < GRAD_STRIPS; ++q) u[q] = uv[min(64 * q + lane, rows - 1)];   // clamped: valid
+    A *g = grad + it.dst;
+    for (int j = 0; j < it.ncol; ++j) {
+        A w = wv[j];
+        if (it.kind != 0) w = static_cast<A>(ldexp(static_cast<double>(w), sg * re[j]));
+        A *gc = g + j * ld;
+#pragma unroll
+        for (int q = 0; q < GRAD_STRIPS; ++q) {
+            const int r = 64 * q + lane;
+            if (64 * q < rows && r < rows) gc[r] = u[q] * w;
         }
-        grad[it.dst + e] = v;
     }
 }
 
-// 0.5 * sum(res^2) into *out (double), one workgroup, fixed order.
+// 0.5 * sum(res^2) into *out (double) in two fixed-order passes: LOSS_PARTS
+// workgroups (at most) sum contiguous ranges into part[], one workgroup
+// adds them.
 template <class A>
-__global__ __launch_bounds__(256) void k_loss(const A *__restrict__ res, int64_t n,
-                                              double *__restrict__ out) {
-    __shared__ double part[256];
+__global__ __launch_bounds__(256) void k_loss_part(const A *__restrict__ res, int64_t n, int64_t per,
+                                                   double *__restrict__ part) {
+    __shared__ double sh[256];
+    const int64_t lo = static_cast<int64_t>(blockIdx.x) * per;
+    const int64_t hi = min(n, lo + per);
     double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) {
-        const double v = static_cast<double>(res[i]);
-        acc += v * v;
+    for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += 256 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int64_t i = i0 + 256 * u;
+            v[u] = i < hi ? static_cast<double>(res[i]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u] * v[u];
     }
-    part[threadIdx.x] = acc;
+    sh[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = 0.5 * part[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_loss_final(const double *__restrict__ part, int nparts,
+                                                    double *__restrict__ out) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) acc += part[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = 0.5 * sh[0];
 }
 
 template <class A>
 static hipError_t run_grads(const Plan &p, const void *x, const void *res, void *grad,
                             bool want_loss, hipStream_t s) {
     if (p.ngrad > 0 && grad) {
-        hipLaunchKernelGGL(k_grad<A>, dim3(static_cast<unsigned>(p.ngrad)), dim3(256), 0, s, p.grad,
-                           p.rexp, static_cast<const A *>(x), static_cast<const A *>(res),
+        hipLaunchKernelGGL(k_grad<A>, dim3(static_cast<unsigned>((p.ngrad + WAVES - 1) / WAVES)), dim3(64 * WAVES),
+                           0, s, p.grad, p.ngrad, p.rexp, static_cast<const A *>(x), static_cast<const A *>(res),
                            static_cast<const A *>(p.fwd.t), static_cast<const A *>(p.bwd.t),
                            static_cast<A *>(grad));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (want_loss) {
-        hipLaunchKernelGGL(k_loss<A>, dim3(1), dim3(256), 0, s, static_cast<const A *>(res), p.nrows,
-                           static_cast<double *>(p.lossbuf));
+        // lossbuf: [loss | LOSS_PARTS partial sums]; the split depends on n only
+        double *lb = static_cast<double *>(p.lossbuf);
+        const int64_t n = p.nrows;
+        const int64_t parts = n > 0 ? std::min<int64_t>(LOSS_PARTS, (n + 2047) / 2048) : 0;
+        if (parts > 0) {
+            const int64_t per = (n + parts - 1) / parts;
+            hipLaunchKernelGGL(k_loss_part<A>, dim3(static_cast<unsigned>(parts)), dim3(256), 0, s,
+                               static_cast<const A *>(res), n, per, lb + 1);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, s, static_cast<const double *>(lb + 1),
+                           static_cast<int>(parts), lb);
         return hipGetLastError();
     }
     return hipSuccess;
@@ -288,7 +365,7 @@ static hipError_t run(const Plan &p, const Direction &d, const void *xv, void *y
         // cover a whole leaf write t directly, chunked items write partials
         // (their offsets already include tLen) and k_reduce folds them.
         const unsigned g = static_cast<unsigned>((d.nproj + WAVES - 1) / WAVES);
-        hipLaunchKernelGGL(k_proj<St>, dim3(g), dim3(64 * WAVES), 0, s, d.blk, d.proj, d.nproj,
+        hipLaunchKernelGGL(k_proj<St>, dim3(g), dim3(64 * WAVES), 0, s, d.proj, d.nproj,
                            p.fac, p.fac_lo, x, t);
         e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -596,10 +596,11 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
             }
             for (int64_t c = 0; c < nchunk; ++c) {
                 ProjItem it{};
-                it.blk = static_cast<int32_t>(b);
-                it.row0 = static_cast<int32_t>(c * chunk);
-                it.rows = static_cast<int32_t>(std::min<int64_t>(chunk, o.n - c * chunk));
-                it.j0 = static_cast<int32_t>(j0);
+                const int64_t row0 = c * chunk;
+                it.base = o.offB + j0 * o.n + row0;
+                it.xoff = o.col0 + row0;
+                it.ld = static_cast<int32_t>(o.n);
+                it.rows = static_cast<int32_t>(std::min<int64_t>(chunk, o.n - row0));
                 it.kc = kc;
                 it.out = nchunk > 1 ? tLen + pOff + c * kc : tDst;
                 proj.push_back(it);
@@ -686,6 +687,7 @@ void free_plan(hm::Plan *p) {
     DeviceGuard g(p->device);
     for (hipEvent_t e : p->timing.ev) (void)hipEventDestroy(e);
     for (void *a : p->allocs) (void)hipFree(a);
+    if (p->losshost) (void)hipHostFree(p->losshost);
     delete p;
 }
 
@@ -941,29 +943,38 @@ int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
         if (!(p->flags & HM_NO_TRANSPOSE)) build_direction(*p, p->bwd, ncols, nrows, bw, rkv.data(), dT.data());
         // adjoint gradient items over the caller's own arena offsets
         {
-            constexpr int64_t GRAD_CHUNK = 16384;
+            // items of up to GRAD_STRIPS 64-row strips and about 64 stores per lane
             std::vector<GradItem> gi;
-            auto add = [&](int kind, int64_t dst, int64_t len, int64_t ld, int64_t vec0, int64_t aux) {
-                for (int64_t e0 = 0; e0 < len; e0 += GRAD_CHUNK)
-                    gi.push_back(GradItem{dst, vec0, aux, e0, std::min(len, e0 + GRAD_CHUNK),
-                                          static_cast<int32_t>(ld), kind});
+            auto add = [&](int kind, int64_t dst, int64_t ld, int64_t ncol, int64_t vec0, int64_t aux) {
+                const int64_t S = std::min<int64_t>(GRAD_STRIPS, (ld + 63) / 64);
+                const int64_t R = 64 * S, C = std::max<int64_t>(1, 64 / S);
+                for (int64_t i0 = 0; i0 < ld; i0 += R)
+                    for (int64_t j0 = 0; j0 < ncol; j0 += C)
+                        gi.push_back(GradItem{dst + i0 + j0 * ld, vec0 + i0, aux + j0, static_cast<int32_t>(ld),
+                                              static_cast<int32_t>(std::min(R, ld - i0)),
+                                              static_cast<int32_t>(std::min(C, ncol - j0)), kind});
             };
             int64_t tpos = 0;
             for (int64_t b = 0; b < nblocks; ++b) {
                 const int64_t *r = blocks + 8 * b;   // caller's offsets
                 const int64_t m = r[3], n = r[4], k = r[5];
                 if (r[0] == KIND_DENSE) {
-                    if (m > 0 && n > 0) add(0, r[6], m * n, m, r[1], r[2]);
+                    if (m > 0 && n > 0) add(0, r[6], m, n, r[1], r[2]);
                 } else if (k > 0) {
-                    if (m > 0) add(1, r[6], m * k, m, r[1], tpos);
-                    if (n > 0) add(2, r[7], n * k, n, r[2], tpos);
+                    if (m > 0) add(1, r[6], m, k, r[1], tpos);
+                    if (n > 0) add(2, r[7], n, k, r[2], tpos);
                     tpos += k;
                 }
             }
             p->grad = dev_upload(*p, gi);
             p->ngrad = static_cast<int64_t>(gi.size());
             p->rexp = dev_upload(*p, rexp);
-            p->lossbuf = dev_alloc(*p, sizeof(double));
+            p->lossbuf = dev_alloc(*p, sizeof(double) * (1 + LOSS_PARTS));
+            const hipError_t he = hipHostMalloc(reinterpret_cast<void **>(&p->losshost), sizeof(double));
+            if (he != hipSuccess) {
+                p->losshost = nullptr;
+                throw HipFail{he, "hipHostMalloc"};
+            }
             p->ndata = ndata;
         }
         if (storage != HM_F64 && !(p->flags & HM_NO_BATCHED)) {
@@ -1106,10 +1117,11 @@ int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx
     if (e == hipSuccess) e = hm::launch_matvec(*p, p->bwd, res, dx, 1.0, 0.0, s);
     if (e == hipSuccess) e = hm::launch_adjoint_grads(*p, x, res, grad, loss != nullptr, s);
     if (e == hipSuccess && loss) {
-        double h = 0.0;
-        e = hipMemcpyAsync(&h, p->lossbuf, sizeof(double), hipMemcpyDeviceToHost, s);
+        // read back through the plan's pinned double (a pageable target would
+        // take a staged copy)
+        e = hipMemcpyAsync(p->losshost, p->lossbuf, sizeof(double), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-        *loss = h;
+        *loss = e == hipSuccess ? *p->losshost : 0.0;
     }
     if (e != hipSuccess) return fail(HM_EHIP, std::string("hm_adjoint: ") + hipGetErrorString(e));
     return HM_OK;

@@ -206,19 +206,32 @@ class DeviceHmat:
         _native.check(rc, "hm_graph_create")
         return CapturedProduct(g, self, X, Y)
 
-    def adjoint(self, x, b, grad: bool = True, loss: bool = True, stream=None):
+    def adjoint(self, x, b, grad: bool = True, loss: bool = True, stream=None, out=None):
         """Gradients of L = 0.5*||H x - b||^2 on the device.
 
         Returns ``(L, res, dx, g)``: ``res = H x - b``, ``dx = H' res`` and
         ``g`` the factor gradients in the exported arena layout (``None``
         when ``grad`` is False; ``L`` is ``None`` when ``loss`` is False).
+        ``out = (res, dx, g)`` reuses device buffers from an earlier call
+        (``g`` may be ``None`` when ``grad`` is False): the product writes
+        every element of res and dx and every arena element that belongs to
+        a leaf; a new ``g`` starts as zeros so arena gaps read as zero.
         """
         torch = _torch()
         px = self._vec(x, self.ncols, "x")
         pb = self._vec(b, self.nrows, "b")
-        res = torch.empty(self.nrows, dtype=self.vec_dtype, device=self.device)
-        dx = torch.empty(self.ncols, dtype=self.vec_dtype, device=self.device)
-        g = torch.zeros(self.ndata, dtype=self.vec_dtype, device=self.device) if grad else None
+        if out is not None:
+            res, dx, g = out
+            self._vec(res, self.nrows, "res")
+            self._vec(dx, self.ncols, "dx")
+            if grad:
+                if g is None:
+                    raise ValueError("out: g is None but grad=True")
+                self._vec(g, self.ndata, "g")
+        else:
+            res = torch.empty(self.nrows, dtype=self.vec_dtype, device=self.device)
+            dx = torch.empty(self.ncols, dtype=self.vec_dtype, device=self.device)
+            g = torch.zeros(self.ndata, dtype=self.vec_dtype, device=self.device) if grad else None
         L = ctypes.c_double(0.0)
         rc = _native.lib().hm_adjoint(self._h, px, pb, ctypes.c_void_p(res.data_ptr()),
                                       ctypes.c_void_p(dx.data_ptr()),

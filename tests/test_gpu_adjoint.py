@@ -57,6 +57,8 @@ def test_adjoint_c5_oracle_dense_and_fd(hm, orc, c5):
     L, res, dx, g = d.adjoint(xd, bd)
     Lr, resr, dxr, gr = orc.adjoint(f, x, b)
     dx, g = dx.cpu().numpy(), g.cpu().numpy()
+    assert abs(L - Lr) <= 1e-13 * Lr                # 32 partial sums of 2048 residuals
+    assert relerr(res.cpu().numpy(), resr) <= 1e-13
     assert relerr(dx, dxr) <= 1e-13 and relerr(g, gr) <= 1e-13
     # dense A' gradient on sampled columns: dL/dx_j = sum_i H_ij res_i
     cols = rng.choice(f.ncols, 64, replace=False)
@@ -104,3 +106,29 @@ def test_torch_autograd_x(hm, orc):
     w = torch.from_numpy(np.random.default_rng(4).standard_normal(f.nrows)).cuda()
     (y * w).sum().backward()
     assert relerr(x.grad.cpu().numpy(), orc.matvec_t(f, w.cpu().numpy())) <= 1e-13
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 1e-5)])
+def test_adjoint_reused_buffers_ragged(hm, orc, dtype, tol):
+    """N = 5003 (three uneven loss partitions, leaves of uneven size): a
+    second call into the first call's (now dirty) output buffers, with other
+    inputs, matches the oracle as well."""
+    import torch
+    X = sorted_uniform(5003, 0.0, 30.0, 14)
+    par, c, H = build(hm, X, cauchy, 700, 48, 8)
+    f = hm.flatten(H)
+    d = hm.DeviceHmat(f, dtype)
+    td = torch.float64 if dtype == "f64" else torch.float32
+    rng = np.random.default_rng(15)
+    out = None
+    for _ in range(2):
+        x, b = rng.standard_normal(f.ncols), rng.standard_normal(f.nrows)
+        L, res, dx, g = d.adjoint(torch.from_numpy(x).to("cuda", td), torch.from_numpy(b).to("cuda", td),
+                                  out=out)
+        Lr, resr, dxr, gr = orc.adjoint(f, x, b)
+        assert abs(L - Lr) <= tol * Lr
+        assert relerr(res.cpu().numpy().astype(np.float64), resr) <= tol
+        assert relerr(dx.cpu().numpy().astype(np.float64), dxr) <= tol
+        assert relerr(g.cpu().numpy().astype(np.float64), gr) <= tol
+        out = (res, dx, g)
+    d.close()

@@ -114,3 +114,27 @@ def test_many_leaves_per_strip(hm, orc, dtype, seed):
     V = np.random.default_rng(seed + 5).standard_normal((f.nrows, 32))
     ref = np.stack([orc.matvec_t(f, V[:, i]) for i in range(32)], 1)
     assert relerr(_run(hm, f, dtype, V, transpose=True), ref) <= TOL[dtype]
+
+
+def test_two_plans_two_streams(hm, orc, c1f):
+    """Two plans of one matrix, each on its own default-priority stream, with
+    products interleaved and in flight together: each plan's scratch belongs to
+    its stream, so both results match the oracle."""
+    import torch
+    S = 2
+    plans = [hm.DeviceHmat(c1f, "bf16x2") for _ in range(S)]
+    streams = [torch.cuda.Stream() for _ in range(S)]
+    rng = np.random.default_rng(21)
+    Xs = [rng.standard_normal((c1f.ncols, 32)) for _ in range(S)]
+    Xd = [plans[i].new_cols(c1f.ncols, 32) for i in range(S)]
+    Yd = [plans[i].new_cols(c1f.nrows, 32) for i in range(S)]
+    for i in range(S):
+        Xd[i].copy_(torch.from_numpy(Xs[i]).float())
+    torch.cuda.synchronize()
+    for _ in range(5):
+        for i in range(S):
+            plans[i].matmat(Xd[i], Yd[i], stream=streams[i])
+    torch.cuda.synchronize()
+    for i in range(S):
+        assert relerr(Yd[i].cpu().numpy().astype(np.float64), orc.matmat(c1f, Xs[i])) <= TOL["bf16x2"]
+        plans[i].close()
