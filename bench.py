@@ -134,17 +134,29 @@ def timed_streams(torch, fn, steps: int, warmup: int, streams):
     return e0.elapsed_time(e1) / steps   # ms per step
 
 
-def cpu_baseline(f, seconds: float):
-    """Oracle FP64 matvec, 1 thread, on the same factors; bounded sample."""
+def cpu_baseline(f, seconds: float, threads: int = 1):
+    """Oracle FP64 matvec on the same factors, bounded sample: 1 thread (the
+    serial restatement) or `threads` OpenMP threads (row ownership, the same
+    sums bitwise)."""
     from oracle import oracle
     x = np.random.default_rng(9).standard_normal(f.ncols)
-    oracle.matvec(f, x)        # warm
+    mv = (lambda: oracle.matvec(f, x)) if threads == 1 else (lambda: oracle.matvec_mt(f, x, threads))
+    mv()        # warm
     n, t0 = 0, time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        oracle.matvec(f, x)
+        mv()
         n += 1
     dt = time.perf_counter() - t0
     return n / dt, n, dt
+
+
+def host_threads() -> int:
+    """Threads for the all-core CPU leg: OMP_NUM_THREADS when set (the GPU
+    host's share), else the visible cores, at most 16."""
+    try:
+        return max(1, int(os.environ["OMP_NUM_THREADS"]))
+    except (KeyError, ValueError):
+        return max(1, min(16, os.cpu_count() or 1))
 
 
 def profiled_traffic(storage: str, kernel: str):
@@ -185,8 +197,9 @@ def extras_report(hm, torch, stream, plan_b, Xb, Yb) -> dict:
     "extras"): the transposed batched product H'X on the same plan replicas
     (rotated); the plain
     FP64 H*y at N=65536 (sorted uniform on [0,100], seed 1, Cauchy kernel,
-    MaxBlock 1024, MinBlock 64, MaxRank 16); the FP64 adjoint of
-    L = 0.5||Hx - b||^2 at N=65536 (seed 7, same kernel and parameters)."""
+    MaxBlock 1024, MinBlock 64, MaxRank 16); H\\y on that matrix + 10 I by
+    iterative refinement; the FP64 adjoint of L = 0.5||Hx - b||^2 at
+    N=65536 (seed 7, same kernel and parameters)."""
     import scipy.sparse as sp
     out = {}
     R = len(plan_b)
@@ -218,6 +231,30 @@ def extras_report(hm, torch, stream, plan_b, Xb, Yb) -> dict:
     out["c2_fp64_GBs"] = nbytes / (ms * 1e-3) / 1e9
     log(f"C2 FP64 H*y (N=65536, {f2.nblocks} leaves): {ms * 1e3:.1f} us, {out['c2_fp64_GBs']:.0f} GB/s")
     d2.close()
+
+    # H\y by iterative refinement on the C2 matrix shifted by 10 I (the
+    # reference's solve test shifts its matrix the same way, test/lu.jl:25):
+    # FP64 GPU residuals, GMRES corrections on an FP32 copy
+    par = hm.Params()
+    par.Geom = np.sort(np.random.default_rng(1).uniform(0.0, 100.0, 65536))
+    par.Kernel = lambda x, y: 1.0 / (1.0 + (x - y) ** 2)
+    par.MaxBlock, par.MinBlock, par.MaxRank = 1024, 64, 16
+    par.ConsMethod = "aca2"
+    par.Adj = sp.identity(65536, format="csr")
+    c, H = hm.NewHmat(par)
+    G = H + 10 * hm.I
+    yv = np.random.default_rng(2).standard_normal(65536)
+    hm.solve_ir(G, yv, tol=1e-12, inner="f32")           # plans and warm-up
+    t0 = time.perf_counter()
+    xs, hist = hm.solve_ir(G, yv, tol=1e-12, inner="f32")
+    dt = time.perf_counter() - t0
+    out["c2_solve_ir_ms"] = dt * 1e3
+    out["c2_solve_ir_sweeps"] = len(hist) - 1
+    out["c2_solve_ir_relres"] = hist[-1]
+    log(f"C2+10I solve (FP64 GPU residual, FP32 GMRES corrections; N=65536): {dt * 1e3:.1f} ms, "
+        f"{len(hist) - 1} refinement sweeps, relative residual {hist[-1]:.2e}")
+    for dv in G._device_cache.values():
+        dv.close()
 
     f5 = cauchy_plan(7)
     d5 = hm.DeviceHmat(f5, "f64", {"no_batched": True})
@@ -377,12 +414,17 @@ def main() -> int:
     step_gbs = bytes_b / (ms * 1e-3) / 1e9
     value = nrhs / (ms * 1e-3)
     plain_mv_s = nrhs / (ms64 * 1e-3)
-    cpu = None
+    cpu = cpu_mt = None
     if args.cpu_seconds > 0:
         rate, n, dt = cpu_baseline(f, args.cpu_seconds)
         cpu = {"value": rate, "unit": "H-matvec/s", "cores": 1, "kind": "port",
                "sample": f"{n} FP64 oracle matvecs (1 RHS each) on the same factors in {dt:.1f}s"}
         log(f"cpu baseline: {rate:.3f} matvec/s")
+        nt = host_threads()
+        rate_mt, n_mt, dt_mt = cpu_baseline(f, args.cpu_seconds / 2, nt)
+        cpu_mt = {"value": rate_mt, "unit": "H-matvec/s", "cores": nt, "kind": "port",
+                  "sample": f"{n_mt} FP64 oracle matvecs (1 RHS each, {nt} OpenMP threads) in {dt_mt:.1f}s"}
+        log(f"cpu baseline, {nt} threads: {rate_mt:.3f} matvec/s")
     line = {
         "metric": "H-matvec/s (N=262144, 32 RHS, BF16-split batched, relerr<=1e-5)",
         "value": value,
@@ -419,6 +461,7 @@ def main() -> int:
         "step_GBs": step_gbs,
         "step_frac": step_gbs / HBM_PEAK_GBS,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_mt,
         "relerr_vs_dense_fp64": relerr_dense,
         "relerr_vs_fp64_hmatrix": relerr_h64,
         "plain_fp64": {"H-matvec/s": plain_mv_s, "us_per_matvec": ms64 * 1e3 / nrhs,

@@ -6,8 +6,9 @@
   same warnings on breakdown / non-convergence.
 * ``solve_ir``: ``H\\b`` by mixed-precision iterative refinement.  The
   residual ``r = b - H x`` is accumulated in FP64 on the GPU (the plain FP64
-  product); corrections come from GMRES on a low-precision copy of H (FP32 or
-  split-BF16 storage).  The reference's ``\\`` (src/hlu.jl:553-561) needs an
+  product); corrections come from restarted GMRES on the device (basis
+  vectors in HBM) over a low-precision copy of H (FP32 or split-BF16
+  storage).  The reference's ``\\`` (src/hlu.jl:553-561) needs an
   H-LU factorisation (``lu!``, src/hlu.jl:465-515), which is not part of this
   library; refinement reaches the same solution to the requested tolerance.
 """
@@ -79,16 +80,74 @@ def bicgstab(A, b, x0=None, tol: float = 1e-5, maxiter: Optional[int] = None, M=
     return y
 
 
+def _gmres_device(mv, b, tol: float, restart: int, maxiter: int):
+    """Restarted GMRES on the device: ``mv`` maps a device vector to a new
+    device vector of the same dtype.  Arnoldi with classical Gram-Schmidt
+    applied twice (two matrix-vector products with the basis per step); the
+    (restart+1) x restart Hessenberg matrix and its Givens rotations live on
+    the host, one column (restart+1 numbers) read back per step.  Stops when
+    the residual estimate reaches ``tol * ||b||``.  Returns ``(x, converged)``.
+    """
+    import torch
+    from scipy.linalg import solve_triangular
+    n = b.shape[0]
+    x = torch.zeros_like(b)
+    bnorm = float(torch.linalg.norm(b))
+    if bnorm == 0.0:
+        return x, True
+    target = tol * bnorm
+    V = torch.empty((restart + 1, n), dtype=b.dtype, device=b.device)
+    for _ in range(maxiter):
+        r = b - mv(x) if _ else b.clone()
+        beta = float(torch.linalg.norm(r))
+        if beta <= target:
+            return x, True
+        V[0] = r / beta
+        H = np.zeros((restart + 1, restart))
+        cs, sn = np.zeros(restart), np.zeros(restart)
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        k = restart
+        for j in range(restart):
+            w = mv(V[j])
+            Vj = V[: j + 1]
+            h = Vj @ w
+            w = w - Vj.T @ h
+            h2 = Vj @ w
+            w = w - Vj.T @ h2
+            hn = torch.linalg.norm(w)
+            col = torch.cat([h + h2, hn.reshape(1)]).double().cpu().numpy()
+            H[: j + 2, j] = col
+            if col[-1] > 0:
+                V[j + 1] = w / hn
+            for i in range(j):
+                a, c = H[i, j], H[i + 1, j]
+                H[i, j], H[i + 1, j] = cs[i] * a + sn[i] * c, -sn[i] * a + cs[i] * c
+            d = float(np.hypot(H[j, j], H[j + 1, j]))
+            cs[j], sn[j] = (H[j, j] / d, H[j + 1, j] / d) if d > 0 else (1.0, 0.0)
+            H[j, j], H[j + 1, j] = d, 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            if abs(g[j + 1]) <= target or col[-1] == 0:
+                k = j + 1
+                break
+        y = solve_triangular(H[:k, :k], g[:k])
+        x += V[:k].T @ torch.from_numpy(y).to(b.device, b.dtype)
+        if abs(g[k]) <= target:
+            return x, True
+    return x, float(torch.linalg.norm(b - mv(x))) <= target
+
+
 def solve_ir(H, b, tol: float = 1e-12, inner: str = "f32", inner_tol: float = 1e-4, maxiter: int = 30,
              restart: int = 50):
-    """Solve ``H x = b`` by iterative refinement.
+    """Solve ``H x = b`` by iterative refinement, entirely on the device.
 
     Each sweep: ``r = b - H x`` with the FP64 GPU product, then ``H d = r``
-    by GMRES (relative tolerance ``inner_tol``) on the ``inner`` storage copy
-    of H, then ``x += d``.  Stops when ``||r|| <= tol * ||b||``.
+    by restarted GMRES (relative tolerance ``inner_tol``, at most 20
+    restarts) on the ``inner`` storage copy of H, with the Krylov basis on
+    the device, then ``x += d``.  Stops when ``||r|| <= tol * ||b||``.
     Returns ``(x, residual_history)``.
     """
-    import scipy.sparse.linalg as ssl
     import torch
     from .device import DeviceHmat
     dev64 = H.device("f64") if isinstance(H, Hmat) else H
@@ -103,19 +162,15 @@ def solve_ir(H, b, tol: float = 1e-12, inner: str = "f32", inner_tol: float = 1e
     rd = torch.empty_like(bd)
     nb = float(np.linalg.norm(b))
     hist = []
-    lo = ssl.LinearOperator((N, N), dtype=np.float64,
-                            matvec=lambda v: devlo.matvec(torch.from_numpy(np.asarray(v, np.float64).ravel())
-                                                          .to(devlo.device, devlo.vec_dtype)).cpu().numpy()
-                            .astype(np.float64))
+    mv = devlo.matvec
     for _ in range(maxiter + 1):
         rd.copy_(bd)
         dev64.matvec(xd, rd, alpha=-1.0, beta=1.0)          # r = b - H x, FP64
-        r = rd.cpu().numpy()
-        hist.append(float(np.linalg.norm(r)) / nb if nb > 0 else 0.0)
+        hist.append(float(torch.linalg.norm(rd)) / nb if nb > 0 else 0.0)
         if hist[-1] <= tol:
             break
-        d, info = ssl.gmres(lo, r, rtol=inner_tol, restart=restart, maxiter=20)
-        xd += torch.from_numpy(d).to(xd.device)
+        d, _ok = _gmres_device(mv, rd.to(devlo.vec_dtype), inner_tol, restart, 20)
+        xd += d.to(torch.float64)
     else:
         warnings.warn("iterative refinement did not reach the tolerance")
     return xd.cpu().numpy(), hist

@@ -12,6 +12,8 @@
  *                  rk leaf:    BLAS.gemm!('N','N',s,A,B'*v,1.0,r)  line 344
  *                  leaves in the table's depth-first order = the recursion's
  *                  child order [1,1],[1,2],[2,1],[2,2] (lines 348-351)
+ *   or_matvec_mt the same product on several threads (row ownership),
+ *                bitwise equal to or_matvec
  *   or_matmat    per-column loop     Base.:*(H, X)  src/harithm.jl:317-323
  *   or_matvec_t  r += s*H'*v         hmat_matvec2   src/harithm.jl:357-371
  *                  dense: s*v'*C (359); rk: s*(v'*A)*B' (361)
@@ -74,6 +76,67 @@ void or_matvec(int64_t nblocks, const int64_t *blocks, const double *data, const
             free(t);
         }
     }
+}
+
+/* r += s*H*v on nthreads OpenMP threads, for the CPU baseline's all-core
+ * leg.  Phase 1 computes every low-rank leaf's t = B'v (leaves shared out
+ * dynamically); phase 2 gives each thread a contiguous range of output rows,
+ * and the thread walks the table in order, adding each leaf's contribution
+ * to its own rows.  Every entry of t and every row sum is the same
+ * expression in the same order as in or_matvec, so the result is bitwise
+ * identical to it. */
+void or_matvec_mt(int64_t nblocks, const int64_t *blocks, const double *data, const double *v,
+                  double *r, double s, int64_t nrows, int nthreads) {
+    int64_t tlen = 0;
+    int64_t *toff = (int64_t *)malloc((size_t)(nblocks > 0 ? nblocks : 1) * sizeof(int64_t));
+    for (int64_t q = 0; q < nblocks; ++q) {
+        blk_t b = get(blocks, q);
+        toff[q] = tlen;
+        if (b.kind != KIND_DENSE) tlen += b.k;
+    }
+    double *t = (double *)calloc((size_t)(tlen > 0 ? tlen : 1), sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+    {
+#pragma omp for schedule(dynamic, 8)
+        for (int64_t q = 0; q < nblocks; ++q) {
+            blk_t b = get(blocks, q);
+            if (b.kind == KIND_DENSE) continue;
+            const double *B = data + b.offB;
+            for (int64_t l = 0; l < b.k; ++l) {
+                double acc = 0.0;
+                for (int64_t j = 0; j < b.n; ++j) acc += B[j + l * b.n] * v[b.col0 + j];
+                t[toff[q] + l] = acc;
+            }
+        }
+#pragma omp for schedule(static, 1)
+        for (int64_t th = 0; th < nthreads; ++th) {
+            const int64_t lo = nrows * th / nthreads, hi = nrows * (th + 1) / nthreads;
+            for (int64_t q = 0; q < nblocks; ++q) {
+                blk_t b = get(blocks, q);
+                const int64_t i0 = (lo > b.row0 ? lo : b.row0) - b.row0;
+                const int64_t i1 = (hi < b.row0 + b.m ? hi : b.row0 + b.m) - b.row0;
+                if (i0 >= i1) continue;
+                if (b.kind == KIND_DENSE) {
+                    const double *C = data + b.offA;
+                    for (int64_t i = i0; i < i1; ++i) {
+                        double acc = 0.0;
+                        for (int64_t j = 0; j < b.n; ++j) acc += C[i + j * b.m] * v[b.col0 + j];
+                        r[b.row0 + i] += s * acc;
+                    }
+                } else {
+                    const double *A = data + b.offA;
+                    const double *tb = t + toff[q];
+                    for (int64_t i = i0; i < i1; ++i) {
+                        double acc = 0.0;
+                        for (int64_t l = 0; l < b.k; ++l) acc += A[i + l * b.m] * tb[l];
+                        r[b.row0 + i] += s * acc;
+                    }
+                }
+            }
+        }
+    }
+    free(t);
+    free(toff);
 }
 
 /* R[:, c] += s*H*X[:, c] for each column c */

@@ -36,6 +36,8 @@ def lib():
         L = ctypes.CDLL(_SO)
         L.or_matvec.argtypes = [_I, _P, _P, _P, _P, _D]
         L.or_matvec.restype = None
+        L.or_matvec_mt.argtypes = [_I, _P, _P, _P, _P, _D, _I, ctypes.c_int]
+        L.or_matvec_mt.restype = None
         L.or_matmat.argtypes = [_I, _P, _P, _I, _P, _I, _P, _I, _D]
         L.or_matmat.restype = None
         L.or_matvec_t.argtypes = [_I, _P, _P, _P, _P, _D]
@@ -65,6 +67,16 @@ def matvec(f, v, r=None, s: float = 1.0) -> np.ndarray:
     assert v.shape == (f.ncols,)
     r = np.zeros(f.nrows) if r is None else np.array(r, dtype=np.float64, copy=True)
     lib().or_matvec(b.shape[0], _p(b), _p(d), _p(v), _p(r), s)
+    return r
+
+
+def matvec_mt(f, v, threads: int, r=None, s: float = 1.0) -> np.ndarray:
+    """r += s*H*v on `threads` OpenMP threads; bitwise equal to matvec."""
+    b, d = _arrs(f)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    assert v.shape == (f.ncols,) and threads >= 1
+    r = np.zeros(f.nrows) if r is None else np.array(r, dtype=np.float64, copy=True)
+    lib().or_matvec_mt(b.shape[0], _p(b), _p(d), _p(v), _p(r), s, f.nrows, int(threads))
     return r
 
 

@@ -167,3 +167,32 @@ def test_add_identity(hm):
     G = H + 10 * hm.I
     assert np.allclose(hm.to_fmat(G), hm.to_fmat(H) + 10 * np.eye(500), rtol=0, atol=1e-14)
     assert np.allclose(hm.to_fmat(H), hm.to_fmat(hm.copy_hmat(H)))      # H untouched
+
+
+@pytest.mark.parametrize("restart,dtype", [(60, "float64"), (7, "float64"), (4, "float32")])
+def test_device_gmres_on_cpu_tensors(hm, restart, dtype):
+    """The refinement's GMRES (basis on the tensor's device) run on CPU
+    tensors with a dense operator: it reaches the requested residual with and
+    without restarts, and its answer matches a direct solve."""
+    import torch
+    from hmatrix_amd.solve import _gmres_device
+    rng = np.random.default_rng(restart)
+    n = 300
+    A = 4.0 * np.eye(n) + rng.standard_normal((n, n)) / np.sqrt(n)
+    b = rng.standard_normal(n)
+    td = getattr(torch, dtype)
+    At = torch.from_numpy(A).to(td)
+    calls = []
+
+    def mv(v):
+        calls.append(1)
+        return At @ v
+
+    tol = 1e-10 if dtype == "float64" else 1e-5
+    x, ok = _gmres_device(mv, torch.from_numpy(b).to(td), tol, restart, 200)
+    assert ok
+    x = x.double().numpy()
+    assert np.linalg.norm(b - A @ x) <= 2 * tol * np.linalg.norm(b)
+    assert relerr(x, np.linalg.solve(A, b)) <= 10 * tol
+    if restart < 20:
+        assert len(calls) > restart               # it had to restart

@@ -173,3 +173,17 @@ def test_golden_fixture(hm, orc):
     assert relerr(orc.matmat(f, z["X"]), z["Y"]) < 1e-15
     # the H-matrix (MaxRank 8) approximates the kernel matrix
     assert relerr(z["y"], z["y_dense"]) < 1e-6
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_matvec_bitwise_equal(hm, orc, threads):
+    """The all-core CPU baseline computes exactly the serial oracle's sums:
+    bitwise equal on a hierarchical matrix (rows split unevenly across
+    threads, leaves crossing the splits) and on the hand-made table with a
+    k = 0 leaf, with s != 1 and a nonzero starting r."""
+    par, c, H = build(hm, sorted_uniform(3001, 0, 20, 9), cauchy, 700, 48, 10)
+    rng = np.random.default_rng(threads)
+    for f in (hm.flatten(H), random_flat(hm, threads)):
+        v = rng.standard_normal(f.ncols)
+        r0 = rng.standard_normal(f.nrows)
+        assert np.array_equal(orc.matvec_mt(f, v, threads, r0, -0.75), orc.matvec(f, v, r0, -0.75))
