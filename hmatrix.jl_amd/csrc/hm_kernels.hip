@@ -153,7 +153,10 @@ __global__ __launch_bounds__(256) void k_proj(const ProjItem *__restrict__ items
     }
 }
 
-// Pass 1b: t = sum of chunk partials, chunks in ascending order.
+// Pass 1b: t = sum of chunk partials, chunks in ascending order.  The
+// partials are loaded RED_ROUND at a time (chunks past the last one load
+// chunk 0 and are not added), so a round is one memory round trip.
+constexpr int RED_ROUND = 16;
 template <class A>
 __global__ __launch_bounds__(256) void k_reduce(const RedItem *__restrict__ items, int64_t nitems,
                                                 const A *__restrict__ part, A *__restrict__ t) {
@@ -163,8 +166,19 @@ __global__ __launch_bounds__(256) void k_reduce(const RedItem *__restrict__ item
     if (it >= nitems) return;
     const RedItem r = items[it];
     if (j >= r.kc) return;
+    const A *src = part + r.pOff + j;
     A s = A(0);
-    for (int c = 0; c < r.nchunk; ++c) s += part[r.pOff + static_cast<int64_t>(c) * r.kc + j];
+    for (int c0 = 0; c0 < r.nchunk; c0 += RED_ROUND) {
+        A v[RED_ROUND];
+#pragma unroll
+        for (int u = 0; u < RED_ROUND; ++u) {
+            const int c = c0 + u < r.nchunk ? c0 + u : 0;
+            v[u] = src[static_cast<int64_t>(c) * r.kc];
+        }
+#pragma unroll
+        for (int u = 0; u < RED_ROUND; ++u)
+            if (c0 + u < r.nchunk) s += v[u];
+    }
     t[r.tOff + j] = s;
 }
 

@@ -138,3 +138,25 @@ def test_two_plans_two_streams(hm, orc, c1f):
     for i in range(S):
         assert relerr(Yd[i].cpu().numpy().astype(np.float64), orc.matmat(c1f, Xs[i])) <= TOL["bf16x2"]
         plans[i].close()
+
+
+@pytest.mark.parametrize("dtype", ["bf16x2", "f64"])
+def test_padded_leading_dimensions(hm, orc, c1f, dtype):
+    """X and Y as column-major views with leading dimensions larger than the
+    row count (ldx = n + 37, ldy = n + 5) and 33 right-hand sides (two
+    panels); the padding rows of Y are not written."""
+    import torch
+    d = hm.DeviceHmat(c1f, dtype)
+    td = torch.float64 if dtype == "f64" else torch.float32
+    n, nrhs = c1f.ncols, 33
+    X = np.random.default_rng(33).standard_normal((n, nrhs))
+    Xb = torch.zeros((nrhs, n + 37), dtype=td, device="cuda")
+    Yb = torch.full((nrhs, n + 5), 7.0, dtype=td, device="cuda")
+    Xd, Yd = Xb[:, :n].t(), Yb[:, :n].t()
+    assert Xd.stride() == (1, n + 37) and Yd.stride() == (1, n + 5)
+    Xd.copy_(torch.from_numpy(X).to(td))
+    d.matmat(Xd, Yd)
+    tol = 1e-13 if dtype == "f64" else TOL[dtype]
+    assert relerr(Yd.cpu().numpy().astype(np.float64), orc.matmat(c1f, X)) <= tol
+    assert bool((Yb[:, n:] == 7.0).all())
+    d.close()
