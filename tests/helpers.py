@@ -77,3 +77,29 @@ def comb_flat(hm, seed: int, nrows: int = 41, nblk: int = 70):
                 off += A.size + B.size
         c0 += int(w)
     return hm.FlatHmat(nrows, c0, np.array(blocks, dtype=np.int64), np.concatenate(chunks))
+
+
+def wide_rank_flat(hm, seed: int):
+    """A 3000 x 2900 block table of large leaves with random factors of rank
+    1, 16, 17, 20, 33 and 48: rank columns in several groups of 16, B factors
+    longer than one 1024-row projection chunk, dense leaves between them."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [0, 700, 2100, 3000], [0, 1500, 1600, 2900]
+    ranks = iter(rng.permutation([1, 16, 17, 20, 33, 48]))
+    blocks, chunks, off = [], [], 0
+    for bi in range(3):
+        for bj in range(3):
+            r0, c0 = rows[bi], cols[bj]
+            m, n = rows[bi + 1] - r0, cols[bj + 1] - c0
+            if bi == bj or (bi, bj) == (2, 0):
+                C = rng.standard_normal((m, n)) / np.sqrt(n)
+                blocks.append((0, r0, c0, m, n, 0, off, 0))
+                chunks.append(C.ravel(order="F"))
+                off += C.size
+            else:
+                k = int(next(ranks))
+                A, B = rng.standard_normal((m, k)), rng.standard_normal((n, k)) / np.sqrt(n * k)
+                blocks.append((1, r0, c0, m, n, k, off, off + A.size))
+                chunks += [A.ravel(order="F"), B.ravel(order="F")]
+                off += A.size + B.size
+    return hm.FlatHmat(3000, 2900, np.array(blocks, dtype=np.int64), np.concatenate(chunks))

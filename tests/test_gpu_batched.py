@@ -160,3 +160,16 @@ def test_padded_leading_dimensions(hm, orc, c1f, dtype):
     assert relerr(Yd.cpu().numpy().astype(np.float64), orc.matmat(c1f, X)) <= tol
     assert bool((Yb[:, n:] == 7.0).all())
     d.close()
+
+
+@pytest.mark.parametrize("dtype", ["bf16x2", "f32"])
+def test_wide_ranks_batched(hm, orc, dtype):
+    """Batched H X and H' X on leaves of rank 17..48 (several 16-column
+    groups per leaf in every projection super-tile they cross)."""
+    from tests.helpers import wide_rank_flat
+    f = wide_rank_flat(hm, 2)
+    X = np.random.default_rng(50).standard_normal((f.ncols, 32))
+    assert relerr(_run(hm, f, dtype, X), orc.matmat(f, X)) <= TOL[dtype]
+    V = np.random.default_rng(51).standard_normal((f.nrows, 32))
+    ref = np.stack([orc.matvec_t(f, V[:, i]) for i in range(32)], 1)
+    assert relerr(_run(hm, f, dtype, V, transpose=True), ref) <= TOL[dtype]

@@ -70,3 +70,28 @@ def test_c1_storage_dtypes(hm, orc, c1_aca, dtype, tol):
     x = np.random.default_rng(4).standard_normal(f.ncols)
     y = d.matvec(torch.from_numpy(x).float().cuda()).cpu().numpy()
     assert relerr(y, orc.matvec(f, x)) <= tol
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_wide_ranks_f64_and_f32(hm, orc, seed):
+    """Leaves of rank 17..48 (several 16-column projection groups, each with
+    its own compiled column count) and B factors longer than one projection
+    chunk: H x, H' x and the adjoint in FP64 (<= 1e-13), H x in FP32."""
+    import torch
+    from tests.helpers import wide_rank_flat
+    f = wide_rank_flat(hm, seed)
+    rng = np.random.default_rng(seed + 40)
+    x, v = rng.standard_normal(f.ncols), rng.standard_normal(f.nrows)
+    d = hm.DeviceHmat(f, "f64")
+    xd, vd = torch.from_numpy(x).cuda(), torch.from_numpy(v).cuda()
+    assert relerr(d.matvec(xd).cpu().numpy(), orc.matvec(f, x)) <= TOL64
+    assert relerr(d.matvec(vd, transpose=True).cpu().numpy(), orc.matvec_t(f, v)) <= TOL64
+    L, res, dx, g = d.adjoint(xd, vd)
+    Lr, resr, dxr, gr = orc.adjoint(f, x, v)
+    assert abs(L - Lr) <= TOL64 * Lr
+    assert relerr(dx.cpu().numpy(), dxr) <= TOL64 and relerr(g.cpu().numpy(), gr) <= TOL64
+    d.close()
+    d32 = hm.DeviceHmat(f, "f32")
+    y32 = d32.matvec(torch.from_numpy(x).float().cuda()).cpu().numpy()
+    assert relerr(y32, orc.matvec(f, x)) <= 1e-6
+    d32.close()
