@@ -331,13 +331,15 @@ __global__ __launch_bounds__(256) void kb_reduce(const BRed *__restrict__ red, i
 }
 
 // ---------------------------------------------------------------------------
-// row pass: one wavefront per strip of SROWS rows x NB right-hand sides.
-// No LDS and no barriers: every operand comes from global memory straight
-// into the MFMA registers.  Per K-step a lane loads its 8 A values of strip
-// row r16 (octet g) and, for each right-hand-side block h, the 8 B values
-// of k-rows [8g, 8g + 8) in column 16h + r16: X rows are contiguous in a
-// column of X, panel rows are gathered from t.  KC K-steps are loaded
-// together before their MFMAs.
+// row pass: one wavefront per strip of SROWS rows x NB right-hand sides,
+// SWAVES strips per workgroup.  Per K-step a lane loads its 8 A values of
+// strip row r16 (octet g) and, for each right-hand-side block h, the 8 B
+// values of k-rows [8g, 8g + 8) in column 16h + r16: X rows from the
+// workgroup's X window in LDS (or straight from X when the window is wider
+// than XS_MAX), panel rows gathered from t.  KC K-steps are loaded together
+// before their MFMAs.  The strip record and the first batch's A fragments
+// and t-row indices are loaded before the window's barrier, in flight
+// together with the window's X loads.
 
 // lane's 8 A values of one K-step as BF16 hi / lo vectors
 template <int MODE>
@@ -385,76 +387,43 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
     const int cmin = wgx[2 * wg], W = wgx[2 * wg + 1];
     const float *xzero = xs + NB * XS_LD;
 
-    // Stage X rows [cmin, cmin + W) of all NB columns (zeros past ncols and
-    // ncol): (column, 4-row quad) units, a wavefront reading one column's
-    // rows contiguously; every load issued before the first LDS store.
-    auto stage = [&]() {
-        constexpr int NQ = XS_MAX / 4;                 // quads per column slot
-        constexpr int U = NB * NQ / 256;               // units per thread
-        f4u v[U];
-        int part[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int p = threadIdx.x + 256 * u;
-            const int c = p / NQ, q = p % NQ;
-            const int64_t row = static_cast<int64_t>(cmin) + 4 * q;
-            const bool in = c < ncol && 4 * q < W;
-            part[u] = in ? static_cast<int>(min<int64_t>(4, ncols - row)) : 0;
-            const float *src = part[u] == 4 ? X + static_cast<int64_t>(c) * ldx + row : zeros;
-            v[u] = *reinterpret_cast<const f4u *>(src);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (part[u] > 0 && part[u] < 4) {        // the quad cut by the last row of X
-                const int p = threadIdx.x + 256 * u;
-                const float *src = X + static_cast<int64_t>(p / NQ) * ldx + cmin + 4 * (p % NQ);
-                f4u w = {0.f, 0.f, 0.f, 0.f};
-                for (int e = 0; e < part[u]; ++e) w[e] = src[e];
-                v[u] = w;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int p = threadIdx.x + 256 * u;
-            *reinterpret_cast<float4 *>(&xs[(p / NQ) * XS_LD + 4 * (p % NQ)]) =
-                make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
-        }
-        if (threadIdx.x < 8) xs[NB * XS_LD + threadIdx.x] = 0.0f;
-    };
+    const bool has = s < nstrips;
+    constexpr int NQ = XS_MAX / 4;                 // quads per column slot
+    constexpr int U = NB * NQ / 256;               // units per thread
 
-    auto body = [&](auto staged) {
-    constexpr bool STAGED = decltype(staged)::value;
-    if (s >= nstrips) return;                 // this wavefront has no strip
     floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
     const bool rok0 = r16 < ncol, rok1 = 16 + r16 < ncol;
     const float *xc0 = X + static_cast<int64_t>(r16) * ldx;
     const float *xc1 = X + static_cast<int64_t>(16 + r16) * ldx;
     int row0 = 0, R = 0;
-    uint32_t rec = soff[s];
-    do {
+    // Strip record: its first 128 words in two vector registers (lane l holds
+    // words l and 64 + l; the record arena is padded so these loads stay in
+    // bounds), read with readlane -- one memory round trip for the header
+    // and every K-step descriptor.  A wavefront without a strip reads strip
+    // 0's record (valid addresses, results unused).
+    uint32_t rec = soff[has ? s : 0];
     const uint32_t *rw = recs + rec;
-    // The first 128 words of the record in two vector registers (lane l
-    // holds words l and 64 + l; the record arena is padded so these loads
-    // stay in bounds): header and descriptors are then read with readlane,
-    // one memory round trip for all of them.
-    const uint32_t hv0 = rw[l], hv1 = rw[64 + l];
-    auto word = [&](int i) -> uint32_t {          // i wave-uniform, < 128
+    uint32_t hv0 = rw[l], hv1 = rw[64 + l];
+    int nd = 0, np = 0;
+    uint32_t pidx = 0;
+    auto word = [&](int i) __attribute__((always_inline)) -> uint32_t {
         const uint32_t a = __builtin_amdgcn_readlane(hv0, i & 63);
         const uint32_t b = __builtin_amdgcn_readlane(hv1, i & 63);
         return i < 64 ? a : b;
     };
-    row0 = static_cast<int>(word(0));
-    R = static_cast<int>(word(1));
-    const int nd = static_cast<int>(word(2));        // dense K-steps
-    const int np = static_cast<int>(word(3));        // panel K-steps
-    const uint32_t pidx = word(4);                   // first t-row index block
-    auto desc = [&](int k) {                         // K-step k (dense first, then panel)
+    auto decode = [&]() __attribute__((always_inline)) {
+        row0 = static_cast<int>(word(0));
+        R = static_cast<int>(word(1));
+        nd = static_cast<int>(word(2));
+        np = static_cast<int>(word(3));
+        pidx = word(4);
+    };
+    auto desc = [&](int k) __attribute__((always_inline)) {
         const int i = SREC_HDR + 4 * k;
         return uint4{word(i), word(i + 1), word(i + 2), 0u};
     };
-
     // MFMAs of one K-step: 8 B values per lane and block (hi/lo split here)
-    auto kstep = [&](const bf16x8 &ah, const bf16x8 &al, const f4u (&b)[4]) {
+    auto kstep = [&](const bf16x8 &ah, const bf16x8 &al, const f4u (&b)[4]) __attribute__((always_inline)) {
         bf16x8 xh0, xl0, xh1, xl1;
         {
             const float v[8] = {b[0][0], b[0][1], b[0][2], b[0][3], b[1][0], b[1][1], b[1][2], b[1][3]};
@@ -473,19 +442,18 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
             acc1 = mfma(al, xh1, acc1);
         }
     };
-    // Panel K-step q: A fragments (on == false: the zero buffer) and the t
-    // rows of this lane's octet, indices [8g, 8g + 8) of the 32-word block
-    // at word `ib` of the record.  Every strip has at least one index block
-    // (rows of zeros when it has no panel), so the loads issued never depend
-    // on the strip: the wait counters then never over-wait on a branch.
-    auto load_panel_a = [&](int q, bool on, bf16x8 &ah, bf16x8 &al) {
+    // Panel K-step q: A fragments (on == false: the zero buffer); the t rows
+    // of this lane's octet are indices [8g, 8g + 8) of a 32-word index block
+    // of the record.  Every strip has at least one index block (zero rows
+    // when it has no panel), so the loads issued never depend on the strip.
+    auto load_panel_a = [&](int q, bool on, bf16x8 &ah, bf16x8 &al) __attribute__((always_inline)) {
         const uint4 kd = desc(nd + (on ? q : 0));
         const uint32_t afrag = kd.x, info = kd.z;
         const int rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
         load_a<MODE>(frag, zeros, on && g < nOct && r16 < rows, afrag,
                      (g * rows + r16) * (MODE == MODE_F32 ? 2 : 1), nOct * rows, ah, al);
     };
-    auto gather_t = [&](const uint4 &q0, const uint4 &q1, f4u (&b)[4]) {
+    auto gather_t = [&](const uint4 &q0, const uint4 &q1, f4u (&b)[4]) __attribute__((always_inline)) {
         const uint32_t ix[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -494,36 +462,33 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
             b[2 + (e >> 2)][e & 3] = tr[16];
         }
     };
-
-    // One batch of KC dense K-steps (the first batch also carries the first
-    // panel K-step): all loads first, then the MFMAs.
-    auto batch = [&](int k0, auto first) {
-        constexpr bool FIRST = decltype(first)::value;
-        uint4 q0, q1;
-        bf16x8 pah, pal;
-        if constexpr (FIRST) {
-            const uint4 *ip = reinterpret_cast<const uint4 *>(rw + pidx + 8 * g);
-            q0 = ip[0];
-            q1 = ip[1];
-            load_panel_a(0, np > 0, pah, pal);
-        }
-        bf16x8 ah[KC], al[KC];
-        f4u b[KC][4];       // [h * 2 + half]: B values 8g .. 8g+7 of block h
-        uint4 kd[KC];
-#pragma unroll
-        for (int u = 0; u < KC; ++u)
-            kd[u] = desc(max(0, min(k0 + u, nd - 1)));
+    // A operands of dense K-steps [k0, k0 + KC); K-steps past the strip's end
+    // load the zero buffer, and their MFMAs run anyway (a branch around them
+    // would let the compiler move their loads behind the first wait)
+    auto load_dense_a = [&](int k0, bf16x8 (&ah)[KC], bf16x8 (&al)[KC]) __attribute__((always_inline)) {
 #pragma unroll
         for (int u = 0; u < KC; ++u) {
-            const bool live = k0 + u < nd;                          // wave-uniform
-            const uint32_t afrag = kd[u].x, src = kd[u].y, info = kd[u].z;
+            const uint4 kd = desc(max(0, min(k0 + u, nd - 1)));
+            const bool live = k0 + u < nd;
+            const uint32_t afrag = kd.x, info = kd.z;
             const int roff = info & 15, rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
             const bool aon = live && g < nOct && r16 >= roff && r16 < roff + rows;
             load_a<MODE>(frag, zeros, aon, afrag, (g * rows + r16 - roff) * (MODE == MODE_F32 ? 2 : 1),
                          nOct * rows, ah[u], al[u]);
+        }
+    };
+    // B operands of dense K-steps [k0, k0 + KC): X rows from the LDS window
+    // (16-byte aligned reads) or straight from X
+    auto load_dense_b = [&](int k0, auto staged, f4u (&b)[KC][4]) __attribute__((always_inline)) {
+        constexpr bool STAGED = decltype(staged)::value;
+#pragma unroll
+        for (int u = 0; u < KC; ++u) {
+            const uint4 kd = desc(max(0, min(k0 + u, nd - 1)));
+            const bool live = k0 + u < nd;
+            const uint32_t src = kd.y, info = kd.z;
+            const int nOct = (info >> 9) & 7;
             const bool on = live && g < nOct;
             if constexpr (STAGED) {
-                // B rows from the LDS window (16-byte aligned reads)
                 const int off = static_cast<int>(src) - cmin + 8 * g;
                 const float *p0 = on ? xs + r16 * XS_LD + off : xzero;
                 const float *p1 = on ? xs + (16 + r16) * XS_LD + off : xzero;
@@ -545,61 +510,135 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
                 b[u][3] = reinterpret_cast<const f4u *>(p1)[1];
             }
         }
-        f4u pb[4];
-        if constexpr (FIRST) {
-            // keep the gather behind the loads above: its addresses wait on
-            // the index loads
-            __builtin_amdgcn_sched_barrier(0);
-            gather_t(q0, q1, pb);
-        }
+    };
+    auto batch = [&](int k0, auto staged) __attribute__((always_inline)) {
+        bf16x8 ah[KC], al[KC];
+        f4u b[KC][4];
+        load_dense_a(k0, ah, al);
+        load_dense_b(k0, staged, b);
         __builtin_amdgcn_sched_barrier(0);
-        // K-steps past the strip's end hold zero operands; their MFMAs run
-        // anyway (a branch around them would let the compiler move their
-        // loads behind the first wait)
 #pragma unroll
         for (int u = 0; u < KC; ++u) kstep(ah[u], al[u], b[u]);
-        if constexpr (FIRST) kstep(pah, pal, pb);
     };
-    batch(0, std::true_type{});
-    for (int k0 = KC; k0 < nd; k0 += KC) batch(k0, std::false_type{});
-    for (int q = 1; q < np; ++q) {                 // further panel K-steps
-        bf16x8 ah, al;
-        f4u b[4];
-        load_panel_a(q, true, ah, al);
-        const uint4 *ip = reinterpret_cast<const uint4 *>(rw + desc(nd + q).y + 8 * g);
-        gather_t(ip[0], ip[1], b);
-        kstep(ah, al, b);
-    }
-    rec = word(5);
-    } while (rec != 0u);
-
-    // D layout: lane (g, r16) holds strip rows 4g .. 4g+3 of column 16h + r16
-    const int r = 4 * g;
-    if (r < R) {
+    // rest of a record after its first batch and first panel K-step
+    auto rest = [&](auto staged) __attribute__((always_inline)) {
+        for (int k0 = KC; k0 < nd; k0 += KC) batch(k0, staged);
+        for (int q = 1; q < np; ++q) {
+            bf16x8 ah, al;
+            f4u b[4];
+            load_panel_a(q, true, ah, al);
+            const uint4 *ip = reinterpret_cast<const uint4 *>(rw + desc(nd + q).y + 8 * g);
+            gather_t(ip[0], ip[1], b);
+            kstep(ah, al, b);
+        }
+    };
+    // first batch of a record: loads (issued early), then B reads and MFMAs
+    auto first_loads = [&](uint4 &q0, uint4 &q1, bf16x8 &pah, bf16x8 &pal, bf16x8 (&ah)[KC],
+                           bf16x8 (&al)[KC]) __attribute__((always_inline)) {
+        const uint4 *ip = reinterpret_cast<const uint4 *>(rw + pidx + 8 * g);
+        q0 = ip[0];
+        q1 = ip[1];
+        load_panel_a(0, np > 0, pah, pal);
+        load_dense_a(0, ah, al);
+    };
+    auto first_compute = [&](auto staged, const uint4 &q0, const uint4 &q1, const bf16x8 &pah,
+                             const bf16x8 &pal, const bf16x8 (&ah)[KC], const bf16x8 (&al)[KC]) __attribute__((always_inline)) {
+        f4u b[KC][4];
+        load_dense_b(0, staged, b);
+        f4u pb[4];
+        __builtin_amdgcn_sched_barrier(0);
+        gather_t(q0, q1, pb);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int c = 16 * h + r16;
-            if (c >= ncol) continue;
-            const floatx4 a = h == 0 ? acc0 : acc1;
-            float *dst = Y + static_cast<int64_t>(c) * ldy + row0 + r;
-            f4u v = {alpha * a[0], alpha * a[1], alpha * a[2], alpha * a[3]};
-            if (r + 4 <= R) {
-                if (!beta_zero) v += beta * *reinterpret_cast<const f4u *>(dst);
-                *reinterpret_cast<f4u *>(dst) = v;
-            } else {
+        for (int u = 0; u < KC; ++u) kstep(ah[u], al[u], b[u]);
+        kstep(pah, pal, pb);
+    };
+    auto finish = [&](auto staged) __attribute__((always_inline)) {
+        rest(staged);
+        rec = word(5);
+        while (rec != 0u) {                        // chained records
+            rw = recs + rec;
+            hv0 = rw[l];
+            hv1 = rw[64 + l];
+            decode();
+            uint4 q0, q1;
+            bf16x8 pah, pal, ah[KC], al[KC];
+            first_loads(q0, q1, pah, pal, ah, al);
+            first_compute(staged, q0, q1, pah, pal, ah, al);
+            rest(staged);
+            rec = word(5);
+        }
+        // D layout: lane (g, r16) holds strip rows 4g .. 4g+3 of column 16h + r16
+        const int r = 4 * g;
+        if (r < R) {
 #pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (r + e < R) dst[e] = beta_zero ? v[e] : beta * dst[e] + v[e];
+            for (int h = 0; h < 2; ++h) {
+                const int c = 16 * h + r16;
+                if (c >= ncol) continue;
+                const floatx4 a = h == 0 ? acc0 : acc1;
+                float *dst = Y + static_cast<int64_t>(c) * ldy + row0 + r;
+                f4u v = {alpha * a[0], alpha * a[1], alpha * a[2], alpha * a[3]};
+                if (r + 4 <= R) {
+                    if (!beta_zero) v += beta * *reinterpret_cast<const f4u *>(dst);
+                    *reinterpret_cast<f4u *>(dst) = v;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (r + e < R) dst[e] = beta_zero ? v[e] : beta * dst[e] + v[e];
+                }
             }
         }
-    }
     };
+
     if (W > 0) {
-        stage();
+        // X window loads first (they do not wait for the record), then the
+        // record decode and the first batch's A / index loads, all in flight
+        // together; the LDS stores wait only for the X loads.
+        f4u v[U];
+        int part[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = threadIdx.x + 256 * u;
+            const int c = p / NQ, q = p % NQ;
+            const int64_t row = static_cast<int64_t>(cmin) + 4 * q;
+            const bool in = c < ncol && 4 * q < W;
+            part[u] = in ? static_cast<int>(min<int64_t>(4, ncols - row)) : 0;
+            const float *src = part[u] == 4 ? X + static_cast<int64_t>(c) * ldx + row : zeros;
+            v[u] = *reinterpret_cast<const f4u *>(src);
+        }
+        decode();
+        uint4 q0, q1;
+        bf16x8 pah, pal, ah[KC], al[KC];
+        first_loads(q0, q1, pah, pal, ah, al);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (part[u] > 0 && part[u] < 4) {        // the quad cut by the last row of X
+                const int p = threadIdx.x + 256 * u;
+                const float *src = X + static_cast<int64_t>(p / NQ) * ldx + cmin + 4 * (p % NQ);
+                f4u w = {0.f, 0.f, 0.f, 0.f};
+                for (int e = 0; e < part[u]; ++e) w[e] = src[e];
+                v[u] = w;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = threadIdx.x + 256 * u;
+            *reinterpret_cast<float4 *>(&xs[(p / NQ) * XS_LD + 4 * (p % NQ)]) =
+                make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
+        }
+        if (threadIdx.x < 8) xs[NB * XS_LD + threadIdx.x] = 0.0f;
         __syncthreads();
-        body(std::true_type{});
+        if (!has) return;
+        first_compute(std::true_type{}, q0, q1, pah, pal, ah, al);
+        finish(std::true_type{});
     } else {
-        body(std::false_type{});
+        if (!has) return;
+        decode();
+        uint4 q0, q1;
+        bf16x8 pah, pal, ah[KC], al[KC];
+        first_loads(q0, q1, pah, pal, ah, al);
+        first_compute(std::false_type{}, q0, q1, pah, pal, ah, al);
+        finish(std::false_type{});
     }
 }
 
