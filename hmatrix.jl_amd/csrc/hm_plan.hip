@@ -712,6 +712,13 @@ int fold_timing(hm::Plan *p) {
     return HM_OK;
 }
 
+// Byte ranges [a, a + na*es) and [b, b + nb*es) intersect (empty ranges never do).
+bool overlap(const void *a, int64_t na, const void *b, int64_t nb, size_t es) {
+    if (!a || !b || na <= 0 || nb <= 0) return false;
+    const char *ab = static_cast<const char *>(a), *bb = static_cast<const char *>(b);
+    return ab < bb + nb * es && bb < ab + na * es;
+}
+
 int check_vectors(const hm::Plan *p, const void *x, const void *y) {
     if (!p) return fail(HM_EINVAL, "plan is NULL");
     if ((p->ncols > 0 && !x) || (p->nrows > 0 && !y)) return fail(HM_EINVAL, "NULL vector");
@@ -1106,11 +1113,16 @@ int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx
     if (p->flags & HM_NO_TRANSPOSE) return fail(HM_EUNSUPPORTED, "plan was created with HM_NO_TRANSPOSE");
     if ((p->ncols > 0 && (!x || !dx)) || (p->nrows > 0 && (!b || !res)))
         return fail(HM_EINVAL, "NULL vector");
-    if (res == b || res == x || dx == x || dx == res)
-        return fail(HM_EINVAL, "res and dx must not alias the inputs or each other");
+    const size_t es = acc_size(p->storage);
+    const int64_t m = p->nrows, n = p->ncols;
+    if (overlap(res, m, b, m, es) || overlap(res, m, x, n, es) || overlap(dx, n, x, n, es) ||
+        overlap(dx, n, res, m, es) || overlap(dx, n, b, m, es))
+        return fail(HM_EINVAL, "res and dx must not overlap the inputs or each other");
+    if (grad && (overlap(grad, p->ndata, x, n, es) || overlap(grad, p->ndata, b, m, es) ||
+                 overlap(grad, p->ndata, res, m, es) || overlap(grad, p->ndata, dx, n, es)))
+        return fail(HM_EINVAL, "grad must not overlap the vectors");
     DeviceGuard g(p->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const size_t es = acc_size(p->storage);
     hipError_t e = hipSuccess;
     if (p->nrows > 0) e = hipMemcpyAsync(res, b, static_cast<size_t>(p->nrows) * es, hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hm::launch_matvec(*p, p->fwd, x, res, 1.0, -1.0, s);

@@ -176,7 +176,7 @@ int hm_matmat_t(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx,
  *   loss (host, may be NULL): L; when given the call waits for the stream.
  * Vector and gradient element type is the plan's vector type (double for
  * HM_F64, float otherwise).  All pointers except loss are device pointers;
- * res and dx must not alias x, b or each other.
+ * res, dx and grad must not overlap x, b or each other (HM_EINVAL).
  */
 int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx, void *grad,
                double *loss, void *stream);

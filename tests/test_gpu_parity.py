@@ -95,3 +95,35 @@ def test_wide_ranks_f64_and_f32(hm, orc, seed):
     y32 = d32.matvec(torch.from_numpy(x).float().cuda()).cpu().numpy()
     assert relerr(y32, orc.matvec(f, x)) <= 1e-6
     d32.close()
+
+
+def test_product_argument_errors(hm, c1_aca):
+    """Entry points reject bad arguments with a negative status and a message,
+    before any kernel runs: overlapping x / y, short leading dimensions,
+    negative nrhs, non-finite alpha, and adjoint outputs overlapping its
+    inputs or each other."""
+    import ctypes
+    import torch
+    from hmatrix_amd import _native
+    H, f = c1_aca
+    L = _native.lib()
+    d = hm.DeviceHmat(f, "f64")
+    n = f.nrows
+    buf = torch.zeros(4 * n, dtype=torch.float64, device="cuda")
+    P = lambda off: ctypes.c_void_p(buf.data_ptr() + 8 * off)
+
+    def err(rc, text):
+        assert rc < 0 and text in L.hm_last_error().decode()
+
+    err(L.hm_matvec(d._h, P(0), P(n // 2), 1.0, 0.0, None), "overlap")
+    err(L.hm_matmat(d._h, 2, P(0), n - 1, P(2 * n), n, 1.0, 0.0, None), "leading dimension")
+    err(L.hm_matmat(d._h, -1, P(0), n, P(2 * n), n, 1.0, 0.0, None), "nrhs")
+    err(L.hm_matvec(d._h, P(0), P(2 * n), float("nan"), 0.0, None), "finite")
+    err(L.hm_adjoint(d._h, P(0), P(n), P(n // 2), P(3 * n), None, None, None), "overlap")    # res over x
+    err(L.hm_adjoint(d._h, P(0), P(n), P(2 * n), P(2 * n + 7), None, None, None), "overlap")  # dx over res
+    err(L.hm_adjoint(d._h, P(0), P(n), P(2 * n), P(3 * n), ctypes.c_void_p(buf.data_ptr()), None, None),
+        "grad")
+    # the plan still works after the rejected calls
+    x = torch.from_numpy(np.random.default_rng(6).standard_normal(n)).cuda()
+    assert np.isfinite(d.matvec(x).cpu().numpy()).all()
+    d.close()
