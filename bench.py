@@ -230,6 +230,12 @@ def extras_report(hm, torch, stream, plan_b, Xb, Yb) -> dict:
     out["c2_fp64_matvec_us"] = ms * 1e3
     out["c2_fp64_GBs"] = nbytes / (ms * 1e-3) / 1e9
     log(f"C2 FP64 H*y (N=65536, {f2.nblocks} leaves): {ms * 1e3:.1f} us, {out['c2_fp64_GBs']:.0f} GB/s")
+    # hmat_matvec!(r, H, v, s): r += s*H*v (reads r as well), and v'*H (the transposed layout)
+    ms_acc = timed(torch, lambda i: d2.matvec(x, y, alpha=-1.0, beta=1.0, stream=stream), 50, 5, stream)
+    ms_t = timed(torch, lambda i: d2.matvec(x, y, transpose=True, stream=stream), 50, 5, stream)
+    out["c2_fp64_accumulate_us"] = ms_acc * 1e3
+    out["c2_fp64_matvec_t_us"] = ms_t * 1e3
+    log(f"C2 FP64 r -= H*y: {ms_acc * 1e3:.1f} us; H'*y: {ms_t * 1e3:.1f} us")
     d2.close()
 
     # H\y by iterative refinement on the C2 matrix shifted by 10 I (the

@@ -173,3 +173,41 @@ def test_wide_ranks_batched(hm, orc, dtype):
     V = np.random.default_rng(51).standard_normal((f.nrows, 32))
     ref = np.stack([orc.matvec_t(f, V[:, i]) for i in range(32)], 1)
     assert relerr(_run(hm, f, dtype, V, transpose=True), ref) <= TOL[dtype]
+
+
+def test_two_dimensional_points(hm, orc):
+    """3000 points in the unit square (Lloyd 2-means cluster tree),
+    exp(-|x - y|) with the Euclidean distance, MaxRank 20: the leaves are
+    less regular than in 1-D (ranks up to 20, uneven sizes).  FP64 H x and
+    H' x against the oracle to 1e-13, batched split-BF16 H X to 2e-5."""
+    import scipy.sparse as sp
+    import torch
+    rng = np.random.default_rng(3)
+    par = hm.Params()
+    par.Geom = rng.random((3000, 2))
+    par.Kernel = lambda x, y: np.exp(-np.sqrt(((x - y) ** 2).sum(-1)))
+    par.MaxBlock, par.MinBlock, par.MaxRank = 500, 64, 20
+    par.ConsMethod = "aca2"
+    par.Adj = sp.identity(3000, format="csr")
+    c, H = hm.NewHmat(par)
+    f = hm.flatten(H)
+    x = rng.standard_normal(3000)
+    d = hm.DeviceHmat(f, "f64")
+    xd = torch.from_numpy(x).cuda()
+    assert relerr(d.matvec(xd).cpu().numpy(), orc.matvec(f, x)) <= 1e-13
+    assert relerr(d.matvec(xd, transpose=True).cpu().numpy(), orc.matvec_t(f, x)) <= 1e-13
+    d.close()
+    X = rng.standard_normal((3000, 32))
+    assert relerr(_run(hm, f, "bf16x2", X), orc.matmat(f, X)) <= TOL["bf16x2"]
+
+
+@pytest.mark.parametrize("minblock", [64, 128])
+def test_strong_admissibility_mid_size(hm, orc, minblock):
+    """N = 20000 Cauchy with strong (tridiagonal) admissibility: three dense
+    leaves per block row; with MinBlock 128 the workgroups' X windows exceed
+    the LDS window and the row pass reads X straight from memory."""
+    X = sorted_uniform(20000, 0.0, 60.0, 21)
+    par, c, H = build(hm, X, cauchy, 2048, minblock, 12, adj="strong")
+    f = hm.flatten(H)
+    R = np.random.default_rng(22).standard_normal((f.ncols, 32))
+    assert relerr(_run(hm, f, "bf16x2", R), orc.matmat(f, R)) <= TOL["bf16x2"]
