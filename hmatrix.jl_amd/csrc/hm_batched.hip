@@ -591,9 +591,9 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
     };
 
     if (W > 0) {
-        // X window loads first (they do not wait for the record), then the
-        // record decode and the first batch's A / index loads, all in flight
-        // together; the LDS stores wait only for the X loads.
+        // The record loads (issued above), the X window loads, then -- once
+        // the record has arrived -- the first batch's A and index loads: all
+        // in flight together; the LDS stores wait only for the X loads.
         f4u v[U];
         int part[U];
 #pragma unroll
