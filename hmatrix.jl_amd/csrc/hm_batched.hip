@@ -14,8 +14,9 @@
 //   kb_reduce sums the per-super-tile partials of leaves wider than one
 //             super-tile, in a fixed order (deterministic, no atomics),
 //             largest leaves first.
-//   kb_strips row owner: one wavefront per 16-row strip, four per
-//             workgroup.  A strip is a list of K-steps of 32 columns: each
+//   kb_strips row owner: one wavefront per 16-row strip, 8 or 16 per
+//             workgroup (the plan's row shape).  A strip is a list of
+//             K-steps of 32 columns: each
 //             dense leaf crossing it (A = its rows, B = X rows of its
 //             columns) and a panel holding the A columns of every low-rank
 //             leaf crossing it (B = the matching t rows).  A operands are
@@ -332,11 +333,11 @@ __global__ __launch_bounds__(256) void kb_reduce(const BRed *__restrict__ red, i
 
 // ---------------------------------------------------------------------------
 // row pass: one wavefront per strip of SROWS rows x NB right-hand sides,
-// SWAVES strips per workgroup.  Per K-step a lane loads its 8 A values of
+// SW strips per workgroup.  Per K-step a lane loads its 8 A values of
 // strip row r16 (octet g) and, for each right-hand-side block h, the 8 B
 // values of k-rows [8g, 8g + 8) in column 16h + r16: X rows from the
 // workgroup's X window in LDS (or straight from X when the window is wider
-// than XS_MAX), panel rows gathered from t.  KC K-steps are loaded together
+// than XSM), panel rows gathered from t.  KC K-steps are loaded together
 // before their MFMAs.  The strip record and the first batch's A fragments
 // and t-row indices are loaded before the window's barrier, in flight
 // together with the window's X loads.
@@ -361,8 +362,8 @@ __device__ __forceinline__ void load_a(const void *frag, const float *zeros, boo
     }
 }
 
-template <int MODE, int KC>
-__global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per_xcd,
+template <int MODE, int KC, int SW, int XSM>
+__global__ __launch_bounds__(64 * SW) void kb_strips(int64_t nstrips, int64_t wg_per_xcd,
                                                  const uint32_t *__restrict__ soff,
                                                  const uint32_t *__restrict__ recs,
                                                  const int32_t *__restrict__ wgx,
@@ -374,13 +375,15 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
                                                  int ncol, float alpha, float beta, int beta_zero) {
     // X window of the workgroup as FP32 [column of X][row], then 8 zeros
     // that lanes without data read
+    constexpr int XS_LD = XSM + 4;
+    static_assert((XS_LD / 4) % 2 == 1, "odd 16-byte stride");
     __shared__ float xs[NB * XS_LD + 8];
     // Hardware places workgroup b on XCD b % 8; give each XCD a contiguous
     // range of strips so neighbouring strips share X rows in that L2.
     const int64_t wg = static_cast<int64_t>(blockIdx.x % 8) * wg_per_xcd + blockIdx.x / 8;
-    if (wg * SWAVES >= nstrips) return;       // whole workgroup
+    if (wg * SW >= nstrips) return;           // whole workgroup
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    const int64_t s = wg * SWAVES + wave;
+    const int64_t s = wg * SW + wave;
     const int l = threadIdx.x & 63;
     const int r16 = l & 15;
     const int g = l >> 4;
@@ -388,8 +391,9 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
     const float *xzero = xs + NB * XS_LD;
 
     const bool has = s < nstrips;
-    constexpr int NQ = XS_MAX / 4;                 // quads per column slot
-    constexpr int U = NB * NQ / 256;               // units per thread
+    constexpr int NQ = XSM / 4;                    // quads per column slot
+    constexpr int U = NB * NQ / (64 * SW);         // units per thread
+    static_assert(U * 64 * SW == NB * NQ, "whole units per thread");
 
     floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
     const bool rok0 = r16 < ncol, rok1 = 16 + r16 < ncol;
@@ -598,7 +602,7 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
         int part[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int p = threadIdx.x + 256 * u;
+            const int p = threadIdx.x + 64 * SW * u;
             const int c = p / NQ, q = p % NQ;
             const int64_t row = static_cast<int64_t>(cmin) + 4 * q;
             const bool in = c < ncol && 4 * q < W;
@@ -613,7 +617,7 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (part[u] > 0 && part[u] < 4) {        // the quad cut by the last row of X
-                const int p = threadIdx.x + 256 * u;
+                const int p = threadIdx.x + 64 * SW * u;
                 const float *src = X + static_cast<int64_t>(p / NQ) * ldx + cmin + 4 * (p % NQ);
                 f4u w = {0.f, 0.f, 0.f, 0.f};
                 for (int e = 0; e < part[u]; ++e) w[e] = src[e];
@@ -622,7 +626,7 @@ __global__ __launch_bounds__(256) void kb_strips(int64_t nstrips, int64_t wg_per
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int p = threadIdx.x + 256 * u;
+            const int p = threadIdx.x + 64 * SW * u;
             *reinterpret_cast<float4 *>(&xs[(p / NQ) * XS_LD + 4 * (p % NQ)]) =
                 make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
         }
@@ -666,12 +670,23 @@ static hipError_t run_b(const Direction &d, const void *zeros, int ncol, const f
     }
     if (ev) (void)hipEventRecord(ev[1], s);
     if (b.nstrips > 0) {
-        const int64_t nwg = (b.nstrips + SWAVES - 1) / SWAVES;
+        const int sw = b.shape.sw;
+        const int64_t nwg = (b.nstrips + sw - 1) / sw;
         const int64_t per = (nwg + 7) / 8;        // workgroups per XCD
-        hipLaunchKernelGGL((kb_strips<MODE, SKC>), dim3(static_cast<unsigned>(8 * per)), dim3(64 * SWAVES), 0, s,
-                           b.nstrips, per, b.soff, b.recs, b.wgx, b.frag, static_cast<const float *>(zeros),
-                           static_cast<const float *>(b.t), X, ldx, d.ncols,
-                           Y, ldy, ncol, alpha, beta, beta == 0.0f ? 1 : 0);
+        const dim3 grid(static_cast<unsigned>(8 * per)), block(static_cast<unsigned>(64 * sw));
+        const float *zf = static_cast<const float *>(zeros);
+        const float *tf = static_cast<const float *>(b.t);
+        const int bz = beta == 0.0f ? 1 : 0;
+        if (sw == ROW_WIDE.sw && b.shape.xsm == ROW_WIDE.xsm)
+            hipLaunchKernelGGL((kb_strips<MODE, SKC, ROW_WIDE.sw, ROW_WIDE.xsm>), grid, block, 0, s, b.nstrips,
+                               per, b.soff, b.recs, b.wgx, b.frag, zf, tf, X, ldx, d.ncols, Y, ldy, ncol, alpha,
+                               beta, bz);
+        else if (sw == ROW_NARROW.sw && b.shape.xsm == ROW_NARROW.xsm)
+            hipLaunchKernelGGL((kb_strips<MODE, SKC, ROW_NARROW.sw, ROW_NARROW.xsm>), grid, block, 0, s,
+                               b.nstrips, per, b.soff, b.recs, b.wgx, b.frag, zf, tf, X, ldx, d.ncols, Y, ldy, ncol,
+                               alpha, beta, bz);
+        else
+            return hipErrorInvalidValue;
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

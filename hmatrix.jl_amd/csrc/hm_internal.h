@@ -86,13 +86,20 @@ constexpr int PROJ_SUPER = 256;
 // straight from global memory: X rows [src, src + 32) for a dense piece, or
 // 32 gathered rows of t for the strip's panel of low-rank columns.
 constexpr int SROWS = 16;
-// Strips per workgroup (one wavefront each).
-constexpr int SWAVES = 4;
-// Widest X window (rows of X, all NB columns) a row-pass workgroup stages in
-// LDS, and the LDS row stride: XS_LD / 4 odd keeps the 16-byte reads of 8
-// consecutive lanes (8 columns of X) on distinct bank groups.
-constexpr int XS_MAX = 256;
-constexpr int XS_LD = XS_MAX + 4;
+// Row-pass workgroup shapes: sw strips (one wavefront each) share a window
+// of at most xsm rows of X (all NB columns) staged in LDS with row stride
+// xsm + 4 (stride / 4 odd keeps the 16-byte reads of 8 consecutive lanes, 8
+// columns of X, on distinct bank groups); a workgroup whose dense pieces
+// span more rows reads X straight from memory.  The plan takes the narrow
+// shape (two workgroups per CU) unless more than a tenth of its workgroups
+// would not stage their window; then the wide one (one workgroup of 256
+// rows per CU), for leaves that reach far from the diagonal (strong
+// admissibility).
+struct RowShape {
+    int sw, xsm;
+};
+constexpr RowShape ROW_NARROW = {8, 512};
+constexpr RowShape ROW_WIDE = {16, 1024};
 
 // Strip record, 32-bit words, 16-byte aligned:
 //   0 row0, 1 rows (<= 16), 2 nd = dense K-steps, 3 np = panel K-steps,
@@ -143,6 +150,7 @@ struct BRed {
 struct Batched {
     bool ready = false;
     int64_t nstrips = 0;
+    RowShape shape = ROW_NARROW;  // row-pass workgroup shape
     uint32_t *soff = nullptr;     // word offset of each strip record
     uint32_t *recs = nullptr;     // strip records
     int32_t *wgx = nullptr;       // per workgroup: X window (first row, rows; 0 rows: not staged)

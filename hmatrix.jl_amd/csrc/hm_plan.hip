@@ -196,6 +196,7 @@ void *upload_vec(hm::Plan &p, const std::vector<T> &v) {
 // Host image of the batched layout of one direction.
 struct BatchedHost {
     int64_t nstrips = 0;
+    hm::RowShape shape = hm::ROW_NARROW;
     std::vector<uint32_t> soff, recs;
     std::vector<int32_t> wgx;
     std::vector<uint32_t> pcol;        // projection column descriptors, 4 words each
@@ -309,29 +310,43 @@ bool layout_batched(int mode, int64_t nrows, int64_t ncols, const std::vector<OB
         }
         std::vector<Piece>().swap(ps);
     }
-    // Workgroup X windows: the SWAVES strips of a workgroup read their dense
+    // Workgroup X windows: the sw strips of a workgroup read their dense
     // pieces' X rows from one window [cmin, cmin + W) staged in LDS when the
     // window (pieces started at a multiple of 4, rounded up to octets) is at
-    // most XS_MAX rows wide; otherwise (W = 0) straight from global memory.
-    const int64_t nwg = (nstrips + SWAVES - 1) / SWAVES;
-    std::vector<int32_t> wgx(static_cast<size_t>(2 * nwg), 0);
-    for (int64_t w = 0; w < nwg; ++w) {
-        int64_t cmin = INT64_MAX, cmax = -1;
-        for (int64_t t = w * SWAVES; t < std::min(nstrips, (w + 1) * SWAVES); ++t)
-            for (const Piece &pc : merged[t]) {
-                const int64_t a0 = pc.c0 & ~int64_t(3);
-                cmin = std::min(cmin, a0);
-                cmax = std::max(cmax, a0 + ((pc.c0 + pc.n - a0 + 7) & ~int64_t(7)));
+    // most xsm rows wide; otherwise (W = 0) straight from global memory.
+    // Returns the number of workgroups with dense pieces and no window.
+    std::vector<int32_t> wgx;
+    auto windows = [&](const RowShape &sh) {
+        const int64_t nwg = (nstrips + sh.sw - 1) / sh.sw;
+        wgx.assign(static_cast<size_t>(2 * nwg), 0);
+        int64_t wide = 0;
+        for (int64_t w = 0; w < nwg; ++w) {
+            int64_t cmin = INT64_MAX, cmax = -1;
+            for (int64_t t = w * sh.sw; t < std::min(nstrips, (w + 1) * sh.sw); ++t)
+                for (const Piece &pc : merged[t]) {
+                    const int64_t a0 = pc.c0 & ~int64_t(3);
+                    cmin = std::min(cmin, a0);
+                    cmax = std::max(cmax, a0 + ((pc.c0 + pc.n - a0 + 7) & ~int64_t(7)));
+                }
+            if (cmax > cmin && cmax - cmin <= sh.xsm) {
+                wgx[2 * w] = static_cast<int32_t>(cmin);
+                wgx[2 * w + 1] = static_cast<int32_t>(cmax - cmin);
+            } else if (cmax > cmin) {
+                ++wide;
             }
-        if (cmax > cmin && cmax - cmin <= XS_MAX) {
-            wgx[2 * w] = static_cast<int32_t>(cmin);
-            wgx[2 * w + 1] = static_cast<int32_t>(cmax - cmin);
         }
+        return std::make_pair(wide, nwg);
+    };
+    RowShape shape = ROW_NARROW;
+    const auto nar = windows(ROW_NARROW);
+    if (10 * nar.first > nar.second) {
+        shape = ROW_WIDE;
+        windows(ROW_WIDE);
     }
     for (int64_t t = 0; t < nstrips; ++t) {
         const int64_t srow0 = t * SROWS;
         const int32_t R = static_cast<int32_t>(std::min<int64_t>(SROWS, nrows - srow0));
-        const bool staged = wgx[2 * (t / SWAVES) + 1] > 0;
+        const bool staged = wgx[2 * (t / shape.sw) + 1] > 0;
         const std::vector<Piece> &mg = merged[t];
         std::vector<uint32_t> dsc, pan, idxw;     // dense / panel descriptors, panel index blocks
         int32_t nd = 0, np = 0;
@@ -506,6 +521,7 @@ bool layout_batched(int mode, int64_t nrows, int64_t ncols, const std::vector<OB
     });
     recs.resize(recs.size() + 128, 0u);   // the row pass reads 128 words from every record start
     H.nstrips = nstrips;
+    H.shape = shape;
     H.wgx = std::move(wgx);
     H.soff = std::move(soff);
     H.recs = std::move(recs);
@@ -527,6 +543,7 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
     if (!layout_batched(p.storage, nrows, ncols, obs, fac, dense, H)) return;
     Batched &B = d.bat;
     B.nstrips = H.nstrips;
+    B.shape = H.shape;
     B.soff = dev_upload(p, H.soff);
     B.recs = dev_upload(p, H.recs);
     B.wgx = dev_upload(p, H.wgx);
