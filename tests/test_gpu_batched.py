@@ -103,12 +103,14 @@ def test_kernel_timing_counts_launches(hm, orc, c1f):
 
 
 @pytest.mark.parametrize("dtype", ["bf16x2", "f32", "bf16"])
-@pytest.mark.parametrize("seed", [0, 1])
-def test_many_leaves_per_strip(hm, orc, dtype, seed):
+@pytest.mark.parametrize("seed,nblk", [(0, 70), (1, 70), (2, 200)])
+def test_many_leaves_per_strip(hm, orc, dtype, seed, nblk):
     """More than a hundred narrow leaves cross each 16-row strip: the row
-    pass walks several chained descriptor records per strip."""
+    pass walks several chained descriptor records per strip.  With 200
+    column blocks (about 1500 columns of dense pieces per workgroup) no X
+    window fits in LDS and the row pass reads X straight from memory."""
     from tests.helpers import comb_flat
-    f = comb_flat(hm, seed)
+    f = comb_flat(hm, seed, nblk=nblk)
     X = np.random.default_rng(seed).standard_normal((f.ncols, 32))
     assert relerr(_run(hm, f, dtype, X), orc.matmat(f, X)) <= TOL[dtype]
     V = np.random.default_rng(seed + 5).standard_normal((f.nrows, 32))
