@@ -88,3 +88,34 @@ def test_timing_rejects_null_plan(hm):
     ms = (ctypes.c_double * 3)()
     assert L.hm_plan_timing(None, 1) == -1
     assert L.hm_plan_timing_read(None, ms, 3, None) == -1
+
+
+def _header_arity():
+    """Parameter count of every hm_* function the header declares."""
+    txt = open(os.path.join(ROOT, "include", "hmat_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    out = {}
+    for m in re.finditer(r"(?:int|const char \*)\s*(hm_\w+)\s*\(([^)]*)\)\s*;", txt):
+        params = [p for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
+        out[m.group(1)] = len(params)
+    return out
+
+
+def test_julia_binding_matches_header():
+    """Every entry point the Julia binding names is declared, and each ccall
+    passes as many arguments as the header's declaration takes (also for
+    the calls whose function symbol is chosen at run time)."""
+    ar = _header_arity()
+    jl = open(os.path.join(ROOT, "hmatrix.jl_amd", "julia", "HmatrixAMD.jl")).read()
+    for name in set(re.findall(r":(hm_\w+)", jl)):
+        assert name in ar, name
+    calls = re.findall(r"ccall\(\((:?\w+), libhmat\),\s*\w+,\s*\(([^()]*)\)", jl)
+    assert len(calls) >= 5
+    for fn, types in calls:
+        n = len([t for t in types.split(",") if t.strip()])
+        if fn.startswith(":"):
+            assert ar[fn[1:]] == n, (fn, n, ar[fn[1:]])
+        else:
+            # fn is one of the product entry points with n parameters
+            assert n in (ar["hm_matvec"], ar["hm_matmat"]), (fn, n)
+    assert ar["hm_matvec"] == ar["hm_matvec_t"] and ar["hm_matmat"] == ar["hm_matmat_t"]
