@@ -195,12 +195,12 @@ def c4_report(hm, torch, stream) -> None:
 def extras_report(hm, torch, stream, plan_b, Xb, Yb) -> dict:
     """Timings of the other product forms (stderr and the JSON line's
     "extras"): the transposed batched product H'X on the same plan replicas
-    (rotated); the plain
-    FP64 H*y at N=65536 (sorted uniform on [0,100], seed 1, Cauchy kernel,
-    MaxBlock 1024, MinBlock 64, MaxRank 16); H\\y on that matrix + 10 I by
-    iterative refinement; the batched product on the N=262144 workload with
-    strong admissibility; the FP64 adjoint of L = 0.5||Hx - b||^2 at
-    N=65536 (seed 7, same kernel and parameters)."""
+    (rotated); the plain FP64 H*y, r -= H*y and H'*y at N=65536 (sorted
+    uniform on [0,100], seed 1, Cauchy kernel, MaxBlock 1024, MinBlock 64,
+    MaxRank 16); H\\y on that matrix + 10 I and on the N=262144 matrix
+    + 10 I by iterative refinement; the batched product on the N=262144
+    points with strong admissibility; the FP64 adjoint of
+    L = 0.5||Hx - b||^2 at N=65536 (seed 7, same kernel and parameters)."""
     import scipy.sparse as sp
     out = {}
     R = len(plan_b)
@@ -262,6 +262,28 @@ def extras_report(hm, torch, stream, plan_b, Xb, Yb) -> dict:
         f"{len(hist) - 1} refinement sweeps, relative residual {hist[-1]:.2e}")
     for dv in G._device_cache.values():
         dv.close()
+    # the same at N=262144: exp(-|x-y|) workload + 10 I
+    par3 = hm.Params()
+    par3.Geom = np.sort(np.random.default_rng(3).uniform(0.0, 400.0, 262144))
+    par3.Kernel = lambda x, y: np.exp(-np.abs(x - y))
+    par3.MaxRank, par3.MaxBlock, par3.MinBlock = 24, -1, 64
+    par3.ConsMethod = "aca2"
+    par3.Adj = sp.identity(262144, format="csr")
+    c3, H3 = hm.NewHmat(par3)
+    G3 = H3 + 10 * hm.I
+    y3 = np.random.default_rng(4).standard_normal(262144)
+    hm.solve_ir(G3, y3, tol=1e-12, inner="f32")          # plans and warm-up
+    t0 = time.perf_counter()
+    xs3, hist3 = hm.solve_ir(G3, y3, tol=1e-12, inner="f32")
+    dt3 = time.perf_counter() - t0
+    out["c3_solve_ir_ms"] = dt3 * 1e3
+    out["c3_solve_ir_sweeps"] = len(hist3) - 1
+    out["c3_solve_ir_relres"] = hist3[-1]
+    log(f"C3+10I solve (N=262144, FP64 GPU residual, FP32 GMRES corrections): {dt3 * 1e3:.1f} ms, "
+        f"{len(hist3) - 1} refinement sweeps, relative residual {hist3[-1]:.2e}")
+    for dv in G3._device_cache.values():
+        dv.close()
+    del G3, H3, c3
 
     # the same N=262144 exp(-|x-y|) points with strong (tridiagonal)
     # admissibility: three dense leaves per block row, the batched path with
