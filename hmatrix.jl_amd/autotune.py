@@ -1,10 +1,13 @@
 """Autotuner for the plan shapes of ``hm_config``.
 
-Sweeps ``proj_chunk`` (rows of B per projection work item) and
+``autotune`` sweeps ``proj_chunk`` (rows of B per projection work item) and
 ``row_waves`` (wavefronts per 64-row tile) of the single-right-hand-side
-product, times each candidate plan with HIP events on the current stream and
-returns the fastest.  Results are cached in a JSON file keyed by storage,
-shape and leaf statistics, so a repeated workload skips the sweep.
+product; ``autotune_batched`` sweeps ``strip_shape`` (workgroups of the
+multi-right-hand-side row pass: 8 strips with a 512-row X window, or 16
+with 1024 rows).  Each times every candidate plan with HIP events on the
+current stream and returns the fastest.  Results are cached in a JSON file
+keyed by storage, shape and leaf statistics, so a repeated workload skips
+the sweep.
 """
 from __future__ import annotations
 
@@ -46,6 +49,32 @@ def _time(plan: DeviceHmat, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
+def _load_cache(cache: Optional[str], key: str):
+    if cache and os.path.exists(cache):
+        try:
+            with open(cache) as fh:
+                return json.load(fh).get(key)
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def _store_cache(cache: Optional[str], key: str, entry: dict) -> None:
+    if not cache:
+        return
+    try:
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        data = {}
+        if os.path.exists(cache):
+            with open(cache) as fh:
+                data = json.load(fh)
+        data[key] = entry
+        with open(cache, "w") as fh:
+            json.dump(data, fh, indent=1)
+    except (OSError, ValueError):
+        pass
+
+
 def autotune(flat, dtype: str = "f64", chunks: Iterable[int] = CHUNKS, waves: Iterable[int] = WAVES,
              reps: int = 20, cache: Optional[str] = DEFAULT_CACHE) -> Tuple[dict, Dict[str, float]]:
     """Return ``(best_config, timings_ms)`` for single-right-hand-side products."""
@@ -81,4 +110,45 @@ def autotune(flat, dtype: str = "f64", chunks: Iterable[int] = CHUNKS, waves: It
                 json.dump(data, fh, indent=1)
         except (OSError, ValueError):
             pass
+    return best, timings
+
+
+SHAPES = (1, 2)     # hm_strip_shape: narrow, wide
+
+
+def autotune_batched(flat, dtype: str = "bf16x2", shapes: Iterable[int] = SHAPES, nrhs: int = 32,
+                     reps: int = 20, cache: Optional[str] = DEFAULT_CACHE) -> Tuple[dict, Dict[str, float]]:
+    """Return ``(best_config, timings_ms)`` for ``nrhs``-column products
+    (FP32, BF16 or split-BF16 storage): one plan per ``strip_shape``, each
+    timed over ``reps`` products of the same seeded right-hand sides."""
+    import torch
+    if dtype == "f64":
+        raise ValueError("the batched path needs FP32, BF16 or split-BF16 storage")
+    key = "batched:" + workload_key(flat, dtype) + f":nrhs{nrhs}"
+    hit = _load_cache(cache, key)
+    if hit:
+        return hit["config"], hit["timings"]
+    timings: Dict[str, float] = {}
+    best, best_ms = None, np.inf
+    for sh in shapes:
+        p = DeviceHmat(flat, dtype, {"strip_shape": int(sh), "no_transpose": True})
+        X = p.new_cols(flat.ncols, nrhs)
+        X.copy_(torch.randn(flat.ncols, nrhs, generator=torch.Generator().manual_seed(0)).to(X.device))
+        Y = p.new_cols(flat.nrows, nrhs)
+        for _ in range(3):
+            p.matmat(X, Y)
+        s = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            p.matmat(X, Y)
+        e1.record(s)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        p.close()
+        timings[f"strip_shape{sh}"] = ms
+        if ms < best_ms:
+            best, best_ms = {"strip_shape": int(sh)}, ms
+    _store_cache(cache, key, {"config": best, "timings": timings})
     return best, timings

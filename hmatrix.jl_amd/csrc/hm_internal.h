@@ -219,6 +219,7 @@ struct Plan {
     int storage = 0;
     int proj_chunk = PROJ_CHUNK;   // rows per projection item (single right-hand side)
     int row_waves = WAVES;         // wavefronts per row tile (single right-hand side): 2, 4, 8
+    int strip_shape = 0;           // enum hm_strip_shape requested for the batched row pass
     int flags = 0;
     int64_t nrows = 0, ncols = 0;
     int64_t ndense = 0, nrk = 0, dense_elems = 0, rk_elems = 0, maxrank = 0;

@@ -210,7 +210,7 @@ struct BatchedHost {
 // rk factors come from `fac` (balanced FP64 arena), dense leaves from
 // `dense` at each leaf's offA.  Returns false when the matrix does not fit
 // the layout (the product then runs one right-hand side at a time).
-bool layout_batched(int mode, int64_t nrows, int64_t ncols, const std::vector<OB> &obs,
+bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, const std::vector<OB> &obs,
                     const double *fac, const double *dense, BatchedHost &H) {
     using namespace hm;
     // t offsets
@@ -337,9 +337,9 @@ bool layout_batched(int mode, int64_t nrows, int64_t ncols, const std::vector<OB
         }
         return std::make_pair(wide, nwg);
     };
-    RowShape shape = ROW_NARROW;
-    const auto nar = windows(ROW_NARROW);
-    if (10 * nar.first > nar.second) {
+    RowShape shape = strip_shape == HM_STRIPS_WIDE ? ROW_WIDE : ROW_NARROW;
+    const auto nar = windows(shape);
+    if (strip_shape == HM_STRIPS_AUTO && 10 * nar.first > nar.second) {
         shape = ROW_WIDE;
         windows(ROW_WIDE);
     }
@@ -540,7 +540,7 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
                    const std::vector<OB> &obs, const double *fac, const double *dense) {
     using namespace hm;
     BatchedHost H;
-    if (!layout_batched(p.storage, nrows, ncols, obs, fac, dense, H)) return;
+    if (!layout_batched(p.storage, p.strip_shape, nrows, ncols, obs, fac, dense, H)) return;
     Batched &B = d.bat;
     B.nstrips = H.nstrips;
     B.shape = H.shape;
@@ -847,6 +847,8 @@ int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
     if (c.row_waves != 2 && c.row_waves != 4 && c.row_waves != 8)
         return fail(HM_EINVAL, "row_waves must be 2, 4 or 8");
     if (c.flags & ~(HM_NO_TRANSPOSE | HM_NO_BATCHED)) return fail(HM_EINVAL, "unknown flags");
+    if (c.strip_shape < HM_STRIPS_AUTO || c.strip_shape > HM_STRIPS_WIDE)
+        return fail(HM_EINVAL, "strip_shape must be 0 (auto), 1 (narrow) or 2 (wide)");
     if (nrows < 0 || ncols < 0 || nblocks < 0 || ndata < 0)
         return fail(HM_EINVAL, "negative size");
     if (nrows >= (int64_t(1) << 31) || ncols >= (int64_t(1) << 31))
@@ -915,6 +917,7 @@ int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
     p->storage = storage;
     p->proj_chunk = c.proj_chunk;
     p->row_waves = c.row_waves;
+    p->strip_shape = c.strip_shape;
     p->flags = c.flags;
     p->nrows = nrows;
     p->ncols = ncols;
@@ -1032,6 +1035,13 @@ int hm_plan_destroy(hm_plan_t plan) {
     return HM_OK;
 }
 
+// enum hm_strip_shape of the forward batched row pass (0: no batched layout)
+int64_t strip_shape_in_use(const hm::Plan &p) {
+    const hm::Batched &b = p.fwd.bat;
+    if (b.nstrips == 0) return HM_STRIPS_AUTO;
+    return b.shape.sw == hm::ROW_WIDE.sw ? HM_STRIPS_WIDE : HM_STRIPS_NARROW;
+}
+
 int hm_plan_info(hm_plan_t plan, int64_t *out, int nout) {
     const hm::Plan *p = reinterpret_cast<const hm::Plan *>(plan);
     if (!p || (!out && nout > 0)) return fail(HM_EINVAL, "NULL argument");
@@ -1048,7 +1058,8 @@ int hm_plan_info(hm_plan_t plan, int64_t *out, int nout) {
                                         p->maxrank,
                                         p->proj_chunk,
                                         p->row_waves,
-                                        p->flags};
+                                        p->flags,
+                                        strip_shape_in_use(*p)};
     for (int i = 0; i < nout && i < HM_INFO_NFIELDS; ++i) out[i] = v[i];
     return HM_OK;
 }

@@ -67,6 +67,7 @@ enum hm_info_field {
     HM_INFO_PROJ_CHUNK,      /* hm_config.proj_chunk in effect                   */
     HM_INFO_ROW_WAVES,       /* hm_config.row_waves in effect                    */
     HM_INFO_FLAGS,           /* hm_config.flags                                  */
+    HM_INFO_STRIP_SHAPE,     /* enum hm_strip_shape of the H X row pass in use    */
     HM_INFO_NFIELDS
 };
 
@@ -76,20 +77,28 @@ enum hm_flags {
     HM_NO_BATCHED = 2        /* do not build the multi-right-hand-side layout    */
 };
 
+/* hm_config.strip_shape: workgroups of the multi-right-hand-side row pass */
+enum hm_strip_shape {
+    HM_STRIPS_AUTO = 0,      /* narrow unless a tenth of its X windows overflow */
+    HM_STRIPS_NARROW = 1,    /* 8 strips of 16 rows, X window of <= 512 rows    */
+    HM_STRIPS_WIDE = 2       /* 16 strips of 16 rows, X window of <= 1024 rows  */
+};
+
 /*
  * Plan configuration (zero fields take the defaults).  These are the shapes
  * the autotuner (hmatrix_amd.autotune) sweeps.
- *   proj_chunk: rows of B one projection work item streams in the
- *               single-right-hand-side product (multiple of 64, default 1024)
- *   row_waves:  wavefronts per 64-row tile in the single-right-hand-side
- *               row pass: 2, 4 (default) or 8
- *   flags:      enum hm_flags
+ *   proj_chunk:  rows of B one projection work item streams in the
+ *                single-right-hand-side product (multiple of 64, default 1024)
+ *   row_waves:   wavefronts per 64-row tile in the single-right-hand-side
+ *                row pass: 2, 4 (default) or 8
+ *   flags:       enum hm_flags
+ *   strip_shape: enum hm_strip_shape (multi-right-hand-side row pass)
  */
 typedef struct hm_config {
     int32_t proj_chunk;
     int32_t row_waves;
     int32_t flags;
-    int32_t reserved;
+    int32_t strip_shape;
 } hm_config;
 
 /* products a captured graph replays */

@@ -60,7 +60,8 @@ def test_plan_create_ex_rejects_bad_config(hm):
     L = _native.lib()
     b = np.array([[0, 0, 0, 1, 1, 0, 0, 0]], dtype=np.int64)
     d = np.ones(1)
-    for cfg in [_native.HmConfig(100, 4, 0, 0), _native.HmConfig(1024, 3, 0, 0), _native.HmConfig(1024, 4, 8, 0)]:
+    for cfg in [_native.HmConfig(100, 4, 0, 0), _native.HmConfig(1024, 3, 0, 0), _native.HmConfig(1024, 4, 8, 0),
+                _native.HmConfig(1024, 4, 0, 3), _native.HmConfig(1024, 4, 0, -1)]:
         h = ctypes.c_void_p()
         rc = L.hm_plan_create_ex(1, 1, 1, b.ctypes.data_as(ctypes.c_void_p), d.ctypes.data_as(ctypes.c_void_p),
                                  1, 0, ctypes.byref(cfg), ctypes.byref(h))

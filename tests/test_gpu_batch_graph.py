@@ -106,3 +106,32 @@ def test_autotune_picks_a_config(hm, tmp_path):
     assert best["proj_chunk"] in (256, 1024) and best["row_waves"] in (2, 4) and len(t) == 4
     best2, t2 = autotune(f, "f64", chunks=(256, 1024), waves=(2, 4), reps=3, cache=str(tmp_path / "at.json"))
     assert best2 == best and t2 == t     # served from the cache
+
+
+def test_strip_shapes_agree_and_autotune(hm, orc, tmp_path):
+    """Both multi-right-hand-side row-pass shapes, forced through
+    hm_config.strip_shape, match the oracle (strong admissibility, so the
+    narrow shape reads some X windows straight from memory); plan info
+    reports the shape in use; the batched autotuner times both and caches."""
+    import torch
+    from hmatrix_amd.autotune import autotune_batched
+    from tests.helpers import build, cauchy, sorted_uniform
+    X = sorted_uniform(12000, 0.0, 40.0, 31)
+    par, c, H = build(hm, X, cauchy, 2048, 128, 10, adj="strong")
+    f = hm.flatten(H)
+    R = np.random.default_rng(32).standard_normal((f.ncols, 32))
+    ref = orc.matmat(f, R)
+    for shape in (1, 2):
+        d = hm.DeviceHmat(f, "bf16x2", {"strip_shape": shape})
+        assert d.info()["strip_shape"] == shape
+        Xd = d.new_cols(f.ncols, 32)
+        Xd.copy_(torch.from_numpy(R).float())
+        Y = d.matmat(Xd).cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(Y - ref) / np.linalg.norm(ref) <= 2e-5
+        d.close()
+    auto = hm.DeviceHmat(f, "bf16x2")
+    assert auto.info()["strip_shape"] in (1, 2)
+    auto.close()
+    best, t = autotune_batched(f, "bf16x2", reps=3, cache=str(tmp_path / "atb.json"))
+    assert best["strip_shape"] in (1, 2) and len(t) == 2
+    assert autotune_batched(f, "bf16x2", reps=3, cache=str(tmp_path / "atb.json")) == (best, t)
