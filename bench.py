@@ -364,7 +364,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the product is "
-              "not sharded across GPUs in this build", file=sys.stderr)
+              "not sharded across GPUs", file=sys.stderr)
         return 2
 
     import torch

@@ -49,7 +49,7 @@ enum hm_status {
     HM_EINVAL = -1,       /* bad argument, inconsistent block table          */
     HM_EHIP = -2,         /* HIP runtime error                               */
     HM_ENOMEM = -3,       /* device allocation failed                         */
-    HM_EUNSUPPORTED = -4  /* valid request this build does not implement      */
+    HM_EUNSUPPORTED = -4  /* valid request the library does not implement     */
 };
 
 /* fields reported by hm_plan_info, in this order */
