@@ -342,8 +342,8 @@ def extras_report(hm, torch, stream, plan_b, Xb, Yb) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--storage", default="bf16x2", choices=["f32", "bf16", "bf16x2"])
