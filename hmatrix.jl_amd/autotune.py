@@ -79,14 +79,9 @@ def autotune(flat, dtype: str = "f64", chunks: Iterable[int] = CHUNKS, waves: It
              reps: int = 20, cache: Optional[str] = DEFAULT_CACHE) -> Tuple[dict, Dict[str, float]]:
     """Return ``(best_config, timings_ms)`` for single-right-hand-side products."""
     key = workload_key(flat, dtype)
-    if cache and os.path.exists(cache):
-        try:
-            with open(cache) as fh:
-                hit = json.load(fh).get(key)
-            if hit:
-                return hit["config"], hit["timings"]
-        except (OSError, ValueError):
-            pass
+    hit = _load_cache(cache, key)
+    if hit:
+        return hit["config"], hit["timings"]
     timings: Dict[str, float] = {}
     best, best_ms = None, np.inf
     for c in chunks:
@@ -98,18 +93,7 @@ def autotune(flat, dtype: str = "f64", chunks: Iterable[int] = CHUNKS, waves: It
             timings[f"chunk{c}_waves{w}"] = ms
             if ms < best_ms:
                 best, best_ms = {"proj_chunk": int(c), "row_waves": int(w)}, ms
-    if cache:
-        try:
-            os.makedirs(os.path.dirname(cache), exist_ok=True)
-            data = {}
-            if os.path.exists(cache):
-                with open(cache) as fh:
-                    data = json.load(fh)
-            data[key] = {"config": best, "timings": timings}
-            with open(cache, "w") as fh:
-                json.dump(data, fh, indent=1)
-        except (OSError, ValueError):
-            pass
+    _store_cache(cache, key, {"config": best, "timings": timings})
     return best, timings
 
 

@@ -196,3 +196,23 @@ def test_device_gmres_on_cpu_tensors(hm, restart, dtype):
     assert relerr(x, np.linalg.solve(A, b)) <= 10 * tol
     if restart < 20:
         assert len(calls) > restart               # it had to restart
+
+
+def test_autotune_cache_roundtrip(hm, tmp_path):
+    """The autotuners' JSON cache: entries of several workloads coexist, a
+    missing or unreadable file reads as a miss, and the key separates
+    storage, shape and leaf statistics."""
+    from hmatrix_amd.autotune import _load_cache, _store_cache, workload_key
+    path = str(tmp_path / "sub" / "cache.json")
+    assert _load_cache(path, "k") is None
+    _store_cache(path, "a", {"config": {"proj_chunk": 512}, "timings": {"x": 1.0}})
+    _store_cache(path, "b", {"config": {"strip_shape": 2}, "timings": {}})
+    assert _load_cache(path, "a")["config"] == {"proj_chunk": 512}
+    assert _load_cache(path, "b")["config"] == {"strip_shape": 2}
+    with open(path, "w") as fh:
+        fh.write("{not json")
+    assert _load_cache(path, "a") is None
+    par, c, H = build(hm, sorted_uniform(700, 0, 5, 2), cauchy, 200, 32, 6)
+    f = hm.flatten(H)
+    assert workload_key(f, "f64") != workload_key(f, "f32")
+    assert workload_key(f, "f64").startswith("f64:700x700:")
