@@ -52,11 +52,15 @@ def test_graph_replay_matches_eager(hm, orc, members):
     g.close()
 
 
-def test_graph_batched_rhs_and_transpose(hm):
+@pytest.mark.parametrize("shape", [0, 2])
+def test_graph_batched_rhs_and_transpose(hm, shape):
+    """Graph replays of the batched H X and H' X equal eager products
+    bitwise, three times in a row, for the automatic row-pass shape and the
+    wide one (1024-thread workgroups)."""
     import torch
     par, c, H = c1(hm, "aca2")
     f = hm.flatten(H)
-    d = hm.DeviceHmat(f, "bf16x2")
+    d = hm.DeviceHmat(f, "bf16x2", {"strip_shape": shape})
     X = d.new_cols(f.ncols, 32)
     Y = d.new_cols(f.nrows, 32)
     gN = d.graph(X, Y)
