@@ -136,3 +136,23 @@ def autotune_batched(flat, dtype: str = "bf16x2", shapes: Iterable[int] = SHAPES
             best, best_ms = {"strip_shape": int(sh)}, ms
     _store_cache(cache, key, {"config": best, "timings": timings})
     return best, timings
+
+
+def autotune_native(flat, dtype: str = "f64", nrhs: int = 1, reps: int = 20) -> Tuple[dict, float]:
+    """The library's own sweep (``hm_autotune``, the entry point Julia and C
+    callers use): ``nrhs == 1`` sweeps ``proj_chunk`` x ``row_waves``,
+    ``nrhs > 1`` the ``strip_shape``.  Returns ``(config, ms_per_product)``."""
+    import ctypes
+    from . import _native
+    L = _native.lib()
+    b = np.ascontiguousarray(flat.blocks, dtype=np.int64)
+    d = np.ascontiguousarray(flat.data, dtype=np.float64)
+    cfg = _native.HmConfig()
+    ms = ctypes.c_double(0.0)
+    rc = L.hm_autotune(flat.nrows, flat.ncols, b.shape[0], b.ctypes.data_as(ctypes.c_void_p),
+                       d.ctypes.data_as(ctypes.c_void_p), d.size, _native.STORAGE[dtype], int(nrhs), int(reps),
+                       ctypes.byref(cfg), ctypes.byref(ms))
+    _native.check(rc, "hm_autotune")
+    out = {"proj_chunk": cfg.proj_chunk, "row_waves": cfg.row_waves} if nrhs == 1 else \
+        {"strip_shape": cfg.strip_shape}
+    return out, ms.value

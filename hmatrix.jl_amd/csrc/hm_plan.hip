@@ -1305,6 +1305,79 @@ int hm_graph_create(hm_plan_t plan, int op, int64_t nrhs, const void *X, int64_t
     return HM_OK;
 }
 
+int hm_autotune(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *blocks,
+                const double *data, int64_t ndata, int storage, int64_t nrhs, int reps,
+                hm_config *best, double *best_ms) {
+    if (!best) return fail(HM_EINVAL, "best is NULL");
+    if (reps < 1) return fail(HM_EINVAL, "reps must be >= 1");
+    if (nrhs < 1 || nrhs > hm::NB) return fail(HM_EINVAL, "nrhs must be in [1, 32]");
+    if (storage < HM_F64 || storage > HM_BF16X2) return fail(HM_EINVAL, "unknown storage");
+    if (nrhs > 1 && storage == HM_F64)
+        return fail(HM_EUNSUPPORTED, "multi-right-hand-side shapes need FP32, BF16 or split-BF16 storage");
+    std::vector<hm_config> cand;
+    if (nrhs == 1) {
+        for (int c : {256, 512, 1024, 2048, 4096})
+            for (int w : {2, 4, 8}) cand.push_back(hm_config{c, w, HM_NO_TRANSPOSE | HM_NO_BATCHED, 0});
+    } else {
+        for (int sh : {HM_STRIPS_NARROW, HM_STRIPS_WIDE})
+            cand.push_back(hm_config{0, 0, HM_NO_TRANSPOSE, sh});
+    }
+    const size_t es = acc_size(storage);
+    const size_t xb = static_cast<size_t>(std::max<int64_t>(ncols, 1) * nrhs) * es;
+    const size_t yb = static_cast<size_t>(std::max<int64_t>(nrows, 1) * nrhs) * es;
+    void *X = nullptr, *Y = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = HM_OK;
+    double bms = 0.0;
+    hm_config bcfg{};
+    auto hip = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == HM_OK)
+            rc = fail(e == hipErrorOutOfMemory ? HM_ENOMEM : HM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+        return rc == HM_OK;
+    };
+    if (hip(hipMalloc(&X, xb), "hipMalloc") && hip(hipMalloc(&Y, yb), "hipMalloc") &&
+        hip(hipMemset(X, 0, xb), "hipMemset") && hip(hipStreamCreate(&s), "hipStreamCreate") &&
+        hip(hipEventCreate(&e0), "hipEventCreate") && hip(hipEventCreate(&e1), "hipEventCreate")) {
+        for (const hm_config &c : cand) {
+            hm_plan_t p = nullptr;
+            rc = hm_plan_create_ex(nrows, ncols, nblocks, blocks, data, ndata, storage, &c, &p);
+            if (rc != HM_OK) break;
+            auto run = [&]() {
+                return nrhs == 1 ? hm_matvec(p, X, Y, 1.0, 0.0, s)
+                                 : hm_matmat(p, nrhs, X, std::max<int64_t>(ncols, 1), Y,
+                                             std::max<int64_t>(nrows, 1), 1.0, 0.0, s);
+            };
+            for (int i = 0; i < 3 && rc == HM_OK; ++i) rc = run();
+            if (rc == HM_OK && hip(hipEventRecord(e0, s), "hipEventRecord")) {
+                for (int i = 0; i < reps && rc == HM_OK; ++i) rc = run();
+                float ms = 0.0f;
+                if (rc == HM_OK && hip(hipEventRecord(e1, s), "hipEventRecord") &&
+                    hip(hipEventSynchronize(e1), "hipEventSynchronize") &&
+                    hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) {
+                    const double per = static_cast<double>(ms) / reps;
+                    if (bms == 0.0 || per < bms) {
+                        bms = per;
+                        bcfg = c;
+                    }
+                }
+            }
+            hm_plan_destroy(p);
+            if (rc != HM_OK) break;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (X) (void)hipFree(X);
+    if (Y) (void)hipFree(Y);
+    if (rc != HM_OK) return rc;
+    bcfg.flags = 0;
+    *best = bcfg;
+    if (best_ms) *best_ms = bms;
+    return HM_OK;
+}
+
 int hm_graph_launch(hm_graph_t graph, void *stream) {
     if (!graph) return fail(HM_EINVAL, "graph is NULL");
     DeviceGuard gd(graph->device);

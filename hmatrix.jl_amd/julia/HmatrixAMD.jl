@@ -17,7 +17,7 @@ using Hmatrix
 using AMDGPU
 using ChainRulesCore
 
-export GpuHmat, flatten, adjoint_gradients
+export GpuHmat, flatten, adjoint_gradients, autotune
 
 const libhmat = normpath(joinpath(@__DIR__, "..", "libhmat_amd.so"))
 
@@ -161,6 +161,34 @@ function adjoint_gradients(G::GpuHmat, x::ROCVector, b::ROCVector)
                G.handle, pointer(x), pointer(b), pointer(res), pointer(dx), pointer(grad), L, stream_ptr())
     check(rc, "hm_adjoint")
     return L[], res, dx, grad
+end
+
+# struct hm_config (include/hmat_amd.h)
+struct HmConfig
+    proj_chunk::Int32
+    row_waves::Int32
+    flags::Int32
+    strip_shape::Int32
+end
+
+"""
+    autotune(H::Hmat; storage=:f64, nrhs=1, reps=20) -> (config::HmConfig, ms)
+
+The library's sweep of plan shapes (`hm_autotune`): for `nrhs == 1`
+`proj_chunk` x `row_waves`, for `nrhs` in 2:32 (storage :f32, :bf16 or
+:bf16x2) the multi-right-hand-side `strip_shape`.
+"""
+function autotune(H::Hmat; storage::Symbol = :f64, nrhs::Integer = 1, reps::Integer = 20)
+    blocks, data = flatten(H)
+    cfg = Ref(HmConfig(0, 0, 0, 0))
+    ms = Ref{Cdouble}(0)
+    rc = ccall((:hm_autotune, libhmat), Cint,
+               (Int64, Int64, Int64, Ptr{Int64}, Ptr{Float64}, Int64, Cint, Int64, Cint, Ref{HmConfig},
+                Ref{Cdouble}),
+               H.m, H.n, size(blocks, 2), blocks, data, length(data), STORAGE[storage], Int64(nrhs),
+               Cint(reps), cfg, ms)
+    check(rc, "hm_autotune")
+    return cfg[], ms[]
 end
 
 end # module

@@ -123,6 +123,22 @@ int hm_plan_create_ex(int64_t nrows, int64_t ncols, int64_t nblocks,
                       int storage, const hm_config *cfg, hm_plan_t *plan);
 
 /*
+ * Time candidate configurations for one matrix and return the fastest (no
+ * counterpart in the reference: the plan-shape autotuner, for callers that
+ * tune from Julia or C).  nrhs == 1: proj_chunk in {256, 512, 1024, 2048,
+ * 4096} x row_waves in {2, 4, 8} of the single-right-hand-side product;
+ * 2 <= nrhs <= 32 (FP32, BF16 or split-BF16 storage): strip_shape narrow
+ * and wide of the multi-right-hand-side row pass.  Each candidate plan is
+ * created, run 3 times untimed and reps times timed with HIP events on a
+ * stream of the call's own, then destroyed; the call allocates and frees
+ * its input and output buffers (zeros).  best receives the configuration
+ * (flags 0); best_ms (may be NULL) its milliseconds per product.
+ */
+int hm_autotune(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *blocks,
+                const double *data, int64_t ndata, int storage, int64_t nrhs, int reps,
+                hm_config *best, double *best_ms);
+
+/*
  * One plan for nmat independent H-matrices: member i's block table and arena
  * (same format as hm_plan_create) are placed on the diagonal of a
  * (sum nrows) x (sum ncols) matrix, so one product launch evaluates every

@@ -120,3 +120,19 @@ def test_julia_binding_matches_header():
             # fn is one of the product entry points with n parameters
             assert n in (ar["hm_matvec"], ar["hm_matmat"]), (fn, n)
     assert ar["hm_matvec"] == ar["hm_matvec_t"] and ar["hm_matmat"] == ar["hm_matmat_t"]
+
+
+def test_autotune_rejects_before_touching_the_gpu(hm):
+    """hm_autotune validates its arguments first: NULL output, reps < 1,
+    nrhs outside [1, 32], and FP64 storage with several right-hand sides."""
+    from hmatrix_amd import _native
+    L = _native.lib()
+    b = np.array([[0, 0, 0, 1, 1, 0, 0, 0]], dtype=np.int64)
+    d = np.ones(1)
+    bp, dp = b.ctypes.data_as(ctypes.c_void_p), d.ctypes.data_as(ctypes.c_void_p)
+    cfg = _native.HmConfig()
+    assert L.hm_autotune(1, 1, 1, bp, dp, 1, 0, 1, 5, None, None) == -1
+    assert L.hm_autotune(1, 1, 1, bp, dp, 1, 0, 1, 0, ctypes.byref(cfg), None) == -1
+    assert L.hm_autotune(1, 1, 1, bp, dp, 1, 3, 33, 5, ctypes.byref(cfg), None) == -1
+    assert L.hm_autotune(1, 1, 1, bp, dp, 1, 0, 8, 5, ctypes.byref(cfg), None) == -4
+    assert "FP32" in L.hm_last_error().decode()

@@ -139,3 +139,20 @@ def test_strip_shapes_agree_and_autotune(hm, orc, tmp_path):
     best, t = autotune_batched(f, "bf16x2", reps=3, cache=str(tmp_path / "atb.json"))
     assert best["strip_shape"] in (1, 2) and len(t) == 2
     assert autotune_batched(f, "bf16x2", reps=3, cache=str(tmp_path / "atb.json")) == (best, t)
+
+
+def test_native_autotune(hm, orc):
+    """hm_autotune (the C entry point) picks a candidate for one and for 32
+    right-hand sides, and the chosen configuration keeps parity."""
+    import torch
+    from hmatrix_amd.autotune import autotune_native
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    cfg, ms = autotune_native(f, "f64", nrhs=1, reps=3)
+    assert cfg["proj_chunk"] in (256, 512, 1024, 2048, 4096) and cfg["row_waves"] in (2, 4, 8) and ms > 0
+    d = hm.DeviceHmat(f, "f64", cfg)
+    x = np.random.default_rng(5).standard_normal(f.ncols)
+    assert relerr(d.matvec(torch.from_numpy(x).cuda()).cpu().numpy(), orc.matvec(f, x)) <= 1e-13
+    d.close()
+    cfg2, ms2 = autotune_native(f, "bf16x2", nrhs=32, reps=3)
+    assert cfg2["strip_shape"] in (1, 2) and ms2 > 0
