@@ -452,6 +452,11 @@ def main() -> int:
     relerr_dense64 = float(np.linalg.norm(Yp[rows] - Yd) / np.linalg.norm(Yd))
     log(f"plain f64 ({S} streams): {ms64 * 1e3:.1f} us per {nrhs} rhs; relerr vs dense {relerr_dense64:.3e}; "
         f"batched vs plain {relerr_h64:.3e}")
+    # FP64 H*X through hm_matmat: panels of 8 columns share a pass over the
+    # factors (the reference's own precision for a matrix of right-hand sides)
+    ms64m = timed(torch, lambda i: p64.matmat(X64, Y64, stream=stream), args.plain_steps, 1, stream)
+    relerr_m64 = float(np.linalg.norm(Y64.cpu().numpy() - Yp) / np.linalg.norm(Yp))
+    log(f"FP64 H*X ({nrhs} rhs, hm_matmat, one stream): {ms64m * 1e3:.1f} us; vs per-column {relerr_m64:.1e}")
 
     # drop-in path with host buffers (PCIe-inclusive; never the reported value)
     t0 = time.perf_counter()
@@ -535,6 +540,8 @@ def main() -> int:
                        "GB/s": bytes_64 / (ms64 * 1e-3 / nrhs) / 1e9,
                        "relerr_vs_dense_fp64": relerr_dense64},
         "speedup_vs_plain_fp64": value / plain_mv_s,
+        "fp64_matmat": {"us_per_product": ms64m * 1e3, "H-matvec/s": nrhs / (ms64m * 1e-3),
+                        "relerr_vs_per_column": relerr_m64},
         "host_buffer_product_ms": pcie_s * 1e3,
         "algorithmic_bytes_per_step": bytes_b,
     }

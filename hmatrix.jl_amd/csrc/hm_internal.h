@@ -20,6 +20,9 @@ constexpr int PROJ_CHUNK = 1024;
 constexpr int PROJ_KMAX = 16;
 // Waves per workgroup in the generic kernels.
 constexpr int WAVES = 4;
+// Right-hand sides one pass of the single-right-hand-side layout serves
+// when a product has several (FP64 H*X, or plans without the batched layout).
+constexpr int MULTI_RHS = 8;
 // Partial sums of the loss reduction (workgroups of its first pass, at most).
 constexpr int LOSS_PARTS = 256;
 // Size of the plan's zero buffer (covers one t vector and any 16-byte load).
@@ -182,6 +185,7 @@ struct Direction {
     int64_t tLen = 0;      // length of t (one right-hand side)
     int64_t partLen = 0;   // length of the chunk partials
     void *t = nullptr;     // scratch [t (tLen) | partials (partLen)]
+    void *tm = nullptr;    // the same for RM right-hand sides at once, RM values per entry
     Batched bat;
 };
 
@@ -250,6 +254,10 @@ hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void 
 // fwd.t (t = B' x) and bwd.t (s = A' res); optional loss into p.lossbuf.
 hipError_t launch_adjoint_grads(const Plan &p, const void *x, const void *res, void *grad,
                                 bool want_loss, hipStream_t s);
+// Several right-hand sides on the single-right-hand-side layout: one panel
+// of ncol <= MULTI_RHS columns (any storage; vectors of the plan's vector type).
+hipError_t launch_matvec_m(const Plan &p, const Direction &d, int ncol, const void *x, int64_t ldx, void *y,
+                           int64_t ldy, double alpha, double beta, hipStream_t s, hipEvent_t *ev = nullptr);
 // Batched: one panel of ncol <= NB right-hand sides, float vectors.
 hipError_t launch_batched(const Plan &p, const Direction &d, int ncol, const float *X, int64_t ldx,
                           float *Y, int64_t ldy, float alpha, float beta, hipStream_t s,

@@ -157,6 +157,7 @@ __global__ __launch_bounds__(256) void k_proj(const ProjItem *__restrict__ items
 // partials are loaded RED_ROUND at a time (chunks past the last one load
 // chunk 0 and are not added), so a round is one memory round trip.
 constexpr int RED_ROUND = 16;
+constexpr int RM = MULTI_RHS;
 template <class A>
 __global__ __launch_bounds__(256) void k_reduce(const RedItem *__restrict__ items, int64_t nitems,
                                                 const A *__restrict__ part, A *__restrict__ t) {
@@ -243,6 +244,201 @@ __global__ __launch_bounds__(64 * NW) void k_rows(const TileDev *__restrict__ ti
         for (int w = 1; w < NW; ++w) s += red[w][lane];
         const int64_t r = T.row0 + lane;
         y[r] = beta_zero ? alpha * s : beta * y[r] + alpha * s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Several right-hand sides on the single-right-hand-side layout (any storage;
+// the FP64 H*X path): panels of RM columns of X share one pass over the
+// factors.  t holds RM values per rank column (t_m[q*RM + r]); columns of a
+// panel past ncol load column 0 (valid addresses) and are not written.
+
+// Projection of item w, rank columns [j0, j0 + KC): each lane keeps KC x RM
+// accumulators over its rows, then one butterfly per accumulator.
+template <class St, int KC>
+__device__ __forceinline__ void proj_item_m(const ProjItem &w, int j0, const void *__restrict__ fac,
+                                            const void *__restrict__ fac_lo,
+                                            const typename St::acc *__restrict__ x, int64_t ldx, int ncol,
+                                            typename St::acc *__restrict__ out, int lane) {
+    using A = typename St::acc;
+    constexpr int U = KC >= 8 ? 1 : 8 / KC;
+    const int rows = w.rows;
+    const int64_t ld = w.ld;
+    const int64_t base = w.base + static_cast<int64_t>(j0) * ld;
+    int64_t xo[RM];
+#pragma unroll
+    for (int r = 0; r < RM; ++r) xo[r] = w.xoff + static_cast<int64_t>(r < ncol ? r : 0) * ldx;
+    A acc[KC][RM];
+#pragma unroll
+    for (int j = 0; j < KC; ++j)
+#pragma unroll
+        for (int r = 0; r < RM; ++r) acc[j][r] = A(0);
+    for (int r0 = 0; r0 < rows; r0 += 64 * U) {
+        A xv[U][RM];
+        A bv[KC][U];
+        int rc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int row = r0 + 64 * u + lane;
+            rc[u] = min(row, rows - 1);
+#pragma unroll
+            for (int r = 0; r < RM; ++r) {
+                const A v = x[xo[r] + rc[u]];
+                xv[u][r] = row < rows ? v : A(0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < KC; ++j)
+#pragma unroll
+            for (int u = 0; u < U; ++u) bv[j][u] = St::ld(fac, fac_lo, base + j * ld + rc[u]);
+#pragma unroll
+        for (int j = 0; j < KC; ++j)
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int r = 0; r < RM; ++r) acc[j][r] += bv[j][u] * xv[u][r];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int j = 0; j < KC; ++j)
+#pragma unroll
+            for (int r = 0; r < RM; ++r) acc[j][r] += __shfl_xor(acc[j][r], o, 64);
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < KC; ++j)
+#pragma unroll
+            for (int r = 0; r < RM; ++r) out[(w.out + j0 + j) * RM + r] = acc[j][r];
+    }
+}
+
+template <class St>
+__global__ __launch_bounds__(256) void k_proj_m(const ProjItem *__restrict__ items, int64_t nitems,
+                                                const void *__restrict__ fac, const void *__restrict__ fac_lo,
+                                                const typename St::acc *__restrict__ x, int64_t ldx, int ncol,
+                                                typename St::acc *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int64_t it = static_cast<int64_t>(blockIdx.x) * WAVES + wave;
+    if (it >= nitems) return;
+    const ProjItem w = items[it];
+    // columns in groups of at most PGROUP (KC x RM <= 32 accumulators per lane)
+    auto group = [&](int j0, int kc) {
+        switch (kc) {
+            case 1: proj_item_m<St, 1>(w, j0, fac, fac_lo, x, ldx, ncol, out, lane); break;
+            case 2: proj_item_m<St, 2>(w, j0, fac, fac_lo, x, ldx, ncol, out, lane); break;
+            case 3: proj_item_m<St, 3>(w, j0, fac, fac_lo, x, ldx, ncol, out, lane); break;
+            case 4: proj_item_m<St, 4>(w, j0, fac, fac_lo, x, ldx, ncol, out, lane); break;
+            case 5: proj_item_m<St, 5>(w, j0, fac, fac_lo, x, ldx, ncol, out, lane); break;
+            case 6: proj_item_m<St, 6>(w, j0, fac, fac_lo, x, ldx, ncol, out, lane); break;
+            case 7: proj_item_m<St, 7>(w, j0, fac, fac_lo, x, ldx, ncol, out, lane); break;
+            case 8: proj_item_m<St, 8>(w, j0, fac, fac_lo, x, ldx, ncol, out, lane); break;
+            default: break;
+        }
+    };
+    constexpr int PGROUP = 32 / RM;
+    static_assert(PGROUP >= 1 && PGROUP <= 8, "group sizes have compiled cases");
+    for (int j0 = 0; j0 < w.kc; j0 += PGROUP) group(j0, min(w.kc - j0, PGROUP));
+}
+
+// t_m[(tOff + j)*RM + r] = sum of the chunk partials, chunks in ascending order.
+template <class A>
+__global__ __launch_bounds__(256) void k_reduce_m(const RedItem *__restrict__ items, int64_t nitems,
+                                                  const A *__restrict__ part, A *__restrict__ t) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t it = g / (PROJ_KMAX * RM);
+    const int jr = static_cast<int>(g % (PROJ_KMAX * RM));
+    const int j = jr / RM, r = jr % RM;
+    if (it >= nitems) return;
+    const RedItem ri = items[it];
+    if (j >= ri.kc) return;
+    const A *src = part + (ri.pOff + j) * RM + r;
+    const int64_t stride = static_cast<int64_t>(ri.kc) * RM;
+    A s = A(0);
+    for (int c0 = 0; c0 < ri.nchunk; c0 += RED_ROUND) {
+        A v[RED_ROUND];
+#pragma unroll
+        for (int u = 0; u < RED_ROUND; ++u) {
+            const int c = c0 + u < ri.nchunk ? c0 + u : 0;
+            v[u] = src[static_cast<int64_t>(c) * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < RED_ROUND; ++u)
+            if (c0 + u < ri.nchunk) s += v[u];
+    }
+    t[(ri.tOff + j) * RM + r] = s;
+}
+
+// Row owner over the tile's column stream for RM right-hand sides: each
+// stream value meets RM multipliers (x[id + r*ldx] or t_m[~id*RM + r]).
+template <class St, int NW>
+__global__ __launch_bounds__(64 * NW) void k_rows_m(const TileDev *__restrict__ tiles,
+                                                    const int32_t *__restrict__ mult,
+                                                    const void *__restrict__ stream,
+                                                    const void *__restrict__ stream_lo,
+                                                    const typename St::acc *__restrict__ x, int64_t ldx,
+                                                    const typename St::acc *__restrict__ t,
+                                                    typename St::acc *__restrict__ y, int64_t ldy, int ncol,
+                                                    typename St::acc alpha, typename St::acc beta,
+                                                    int beta_zero) {
+    using A = typename St::acc;
+    __shared__ A red[NW][RM][64];
+    const TileDev T = tiles[blockIdx.x];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int R = T.rows;
+    const int ncols = T.c1 - T.c0;
+    const int per = (ncols + NW - 1) / NW;
+    const int cb = T.c0 + min(ncols, wave * per);
+    const int ce = T.c0 + min(ncols, (wave + 1) * per);
+    A a0[RM], a1[RM];
+#pragma unroll
+    for (int r = 0; r < RM; ++r) a0[r] = a1[r] = A(0);
+    const int lr = min(lane, R - 1);
+    int64_t xo[RM];
+#pragma unroll
+    for (int r = 0; r < RM; ++r) xo[r] = static_cast<int64_t>(r < ncol ? r : 0) * ldx;
+    for (int c = cb; c < ce; c += 64) {
+        const int nc = min(64, ce - c);
+        const int32_t id = mult[c + min(lane, nc - 1)];
+        A mv[RM];
+#pragma unroll
+        for (int r = 0; r < RM; ++r)
+            mv[r] = *(id >= 0 ? x + id + xo[r] : t + static_cast<int64_t>(~id) * RM + (r < ncol ? r : 0));
+        const int64_t p0 = T.base + static_cast<int64_t>(c - T.c0) * R + lr;
+        int j = 0;
+        for (; j + 4 <= nc; j += 4) {
+            A v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j + u) * R);
+#pragma unroll
+            for (int r = 0; r < RM; ++r) {
+                a0[r] += v[0] * bcast(mv[r], j);
+                a1[r] += v[1] * bcast(mv[r], j + 1);
+                a0[r] += v[2] * bcast(mv[r], j + 2);
+                a1[r] += v[3] * bcast(mv[r], j + 3);
+            }
+        }
+        for (; j < nc; ++j) {
+            const A v = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j) * R);
+#pragma unroll
+            for (int r = 0; r < RM; ++r) a0[r] += v * bcast(mv[r], j);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RM; ++r) red[wave][r][lane] = a0[r] + a1[r];
+    __syncthreads();
+    if (wave == 0 && lane < R) {
+        const int64_t row = T.row0 + lane;
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+            if (r >= ncol) continue;
+            A s = red[0][r][lane];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) s += red[w][r][lane];
+            A *yp = y + row + static_cast<int64_t>(r) * ldy;
+            *yp = beta_zero ? alpha * s : beta * *yp + alpha * s;
+        }
     }
 }
 
@@ -413,6 +609,54 @@ static hipError_t run(const Plan &p, const Direction &d, const void *xv, void *y
     }
     if (ev) (void)hipEventRecord(ev[2], s);
     return hipSuccess;
+}
+
+template <class St>
+static hipError_t run_m(const Plan &p, const Direction &d, int ncol, const void *xv, int64_t ldx, void *yv,
+                        int64_t ldy, double alpha, double beta, hipStream_t s, hipEvent_t *ev) {
+    using A = typename St::acc;
+    const A *x = static_cast<const A *>(xv);
+    A *y = static_cast<A *>(yv);
+    A *t = static_cast<A *>(d.tm);
+    hipError_t e = hipSuccess;
+    if (d.nproj > 0) {
+        const unsigned g = static_cast<unsigned>((d.nproj + WAVES - 1) / WAVES);
+        hipLaunchKernelGGL(k_proj_m<St>, dim3(g), dim3(64 * WAVES), 0, s, d.proj, d.nproj, p.fac, p.fac_lo, x, ldx,
+                           ncol, t);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if (d.nproj > 0 && d.nred > 0) {
+        const int64_t thr = d.nred * PROJ_KMAX * RM;
+        hipLaunchKernelGGL(k_reduce_m<A>, dim3(static_cast<unsigned>((thr + 255) / 256)), dim3(256), 0, s, d.red,
+                           d.nred, static_cast<const A *>(t) + d.tLen * RM, t);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (ev) (void)hipEventRecord(ev[1], s);
+    if (d.ntiles > 0) {
+        const dim3 grid(static_cast<unsigned>(d.ntiles));
+        const A al = static_cast<A>(alpha), be = static_cast<A>(beta);
+        const int bz = beta == 0.0 ? 1 : 0;
+        hipLaunchKernelGGL((k_rows_m<St, 4>), grid, dim3(256), 0, s, d.tiles, d.mult, d.stream, d.stream_lo, x,
+                           ldx, static_cast<const A *>(t), y, ldy, ncol, al, be, bz);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (ev) (void)hipEventRecord(ev[2], s);
+    return hipSuccess;
+}
+
+hipError_t launch_matvec_m(const Plan &p, const Direction &d, int ncol, const void *x, int64_t ldx, void *y,
+                           int64_t ldy, double alpha, double beta, hipStream_t s, hipEvent_t *ev) {
+    switch (p.storage) {
+        case 0: return run_m<StF64>(p, d, ncol, x, ldx, y, ldy, alpha, beta, s, ev);
+        case 1: return run_m<StF32>(p, d, ncol, x, ldx, y, ldy, alpha, beta, s, ev);
+        case 2: return run_m<StBF16>(p, d, ncol, x, ldx, y, ldy, alpha, beta, s, ev);
+        case 3: return run_m<StBF16x2>(p, d, ncol, x, ldx, y, ldy, alpha, beta, s, ev);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y, double alpha,

@@ -685,6 +685,7 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
     d.tLen = tLen;
     d.partLen = partLen;
     d.t = dev_alloc(p, static_cast<size_t>(tLen + partLen) * acc_size(p.storage));
+    d.tm = dev_alloc(p, static_cast<size_t>(tLen + partLen) * MULTI_RHS * acc_size(p.storage));
 }
 
 struct DeviceGuard {
@@ -788,10 +789,19 @@ int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void 
         }
         return HM_OK;
     }
-    for (int64_t c = 0; c < nrhs; ++c) {
-        const void *xc = static_cast<const char *>(X) + c * ldx * es;
-        void *yc = static_cast<char *>(Y) + c * ldy * es;
-        hipError_t e = hm::launch_matvec(*p, d, xc, yc, alpha, beta, s, record());
+    if (nrhs == 1) {
+        hipError_t e = hm::launch_matvec(*p, d, X, Y, alpha, beta, s, record());
+        if (e != hipSuccess)
+            return fail(HM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        return HM_OK;
+    }
+    // several columns on the single-right-hand-side layout: panels of
+    // MULTI_RHS columns share one pass over the factors
+    for (int64_t c0 = 0; c0 < nrhs; c0 += hm::MULTI_RHS) {
+        const int nc = static_cast<int>(std::min<int64_t>(hm::MULTI_RHS, nrhs - c0));
+        const void *xc = static_cast<const char *>(X) + c0 * ldx * es;
+        void *yc = static_cast<char *>(Y) + c0 * ldy * es;
+        hipError_t e = hm::launch_matvec_m(*p, d, nc, xc, ldx, yc, ldy, alpha, beta, s, record());
         if (e != hipSuccess)
             return fail(HM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
     }

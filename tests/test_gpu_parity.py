@@ -127,3 +127,37 @@ def test_product_argument_errors(hm, c1_aca):
     x = torch.from_numpy(np.random.default_rng(6).standard_normal(n)).cuda()
     assert np.isfinite(d.matvec(x).cpu().numpy()).all()
     d.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", TOL64), ("f32", 1e-6), ("bf16x2", 2e-5)])
+@pytest.mark.parametrize("nrhs", [2, 8, 13])
+def test_multi_rhs_single_layout(hm, orc, dtype, tol, nrhs):
+    """Several right-hand sides on the single-right-hand-side layout (panels of
+    eight columns per pass over the factors; the FP64 H*X path, and FP32 /
+    split-BF16 plans built without the batched layout): H X, H' X and
+    Y = beta Y + alpha H X with padded leading dimensions, against the oracle
+    column by column.  Leaves of rank up to 48 (two groups of columns per
+    projection item) and multi-chunk B factors."""
+    import torch
+    from tests.helpers import wide_rank_flat
+    f = wide_rank_flat(hm, nrhs)
+    d = hm.DeviceHmat(f, dtype, {"no_batched": True})
+    td = torch.float64 if dtype == "f64" else torch.float32
+    rng = np.random.default_rng(nrhs)
+    X = rng.standard_normal((f.ncols, nrhs))
+    Y0 = rng.standard_normal((f.nrows, nrhs))
+    Xb = torch.zeros((nrhs, f.ncols + 3), dtype=td, device="cuda")
+    Yb = torch.zeros((nrhs, f.nrows + 5), dtype=td, device="cuda")
+    Xd, Yd = Xb[:, :f.ncols].t(), Yb[:, :f.nrows].t()
+    Xd.copy_(torch.from_numpy(X).to(td))
+    Yd.copy_(torch.from_numpy(Y0).to(td))
+    d.matmat(Xd, Yd, alpha=0.75, beta=-2.0)
+    ref = -2.0 * Y0 + 0.75 * orc.matmat(f, X)
+    assert relerr(Yd.cpu().numpy().astype(np.float64), ref) <= tol
+    V = rng.standard_normal((f.nrows, nrhs))
+    Vd = d.new_cols(f.nrows, nrhs)
+    Vd.copy_(torch.from_numpy(V).to(td))
+    got = d.matmat(Vd, transpose=True).cpu().numpy().astype(np.float64)
+    ref_t = np.stack([orc.matvec_t(f, V[:, i]) for i in range(nrhs)], 1)
+    assert relerr(got, ref_t) <= tol
+    d.close()
