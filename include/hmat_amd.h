@@ -169,7 +169,11 @@ int hm_matvec(hm_plan_t plan, const void *x, void *y,
 
 /*
  * Y = beta*Y + alpha*H*X for nrhs right-hand sides, column-major X
- * (ncols x nrhs, leading dimension ldx) and Y (nrows x nrhs, ldy).
+ * (ncols x nrhs, leading dimension ldx) and Y (nrows x nrhs, ldy).  FP32,
+ * BF16 and split-BF16 plans run panels of 32 columns on the batched MFMA
+ * layout; FP64 plans (and plans built with HM_NO_BATCHED) panels of 8
+ * columns per pass over the single-right-hand-side layout; one column is
+ * hm_matvec.
  * Replaces: Base.:*(H, X::Matrix) (src/harithm.jl:317-323), which loops
  * hmat_matvec! over the columns.
  */
