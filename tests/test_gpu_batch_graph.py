@@ -156,3 +156,25 @@ def test_native_autotune(hm, orc):
     d.close()
     cfg2, ms2 = autotune_native(f, "bf16x2", nrhs=32, reps=3)
     assert cfg2["strip_shape"] in (1, 2) and ms2 > 0
+
+
+def test_graph_fp64_multi_rhs(hm):
+    """A captured FP64 H X of 10 columns (two panels on the single-right-hand-
+    side layout) replays bitwise equal to the eager product, three times."""
+    import torch
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    d = hm.DeviceHmat(f, "f64")
+    X = d.new_cols(f.ncols, 10)
+    Y = d.new_cols(f.nrows, 10)
+    g = d.graph(X, Y)
+    for rep in range(3):
+        X.copy_(torch.randn(f.ncols, 10, dtype=torch.float64, generator=torch.Generator().manual_seed(rep)))
+        ref = d.matmat(X).clone()
+        torch.cuda.synchronize()
+        Y.fill_(float("nan"))
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(Y, ref), rep
+    g.close()
+    d.close()
