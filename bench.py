@@ -171,7 +171,61 @@ def profiled_traffic(storage: str, kernel: str):
         return None
 
 
-def c4_report(hm, torch, stream) -> None:
+def measure_batched(hm, torch, f, RHS, storage, args, streams, rows, Yd):
+    """Time the batched product in one storage: S products in flight on S
+    streams (the throughput), one at a time (latency), and one at a time
+    with the library's per-kernel events; relative error of 4096 rows
+    against the dense FP64 kernel.  Returns a dict (plans kept under
+    "_plans" for the caller)."""
+    N, nrhs = RHS.shape
+    dev = streams[0].device
+    stream = streams[0]
+    R = args.replicas
+    t0 = time.time()
+    plans = [hm.DeviceHmat(f, storage) for _ in range(R)]
+    info = plans[0].info()
+    log(f"plans ({storage} x{R}) created in {time.time() - t0:.1f}s; info {info}")
+    if info["batched"] != 1:
+        raise RuntimeError("the batched layout was not built for this matrix")
+    Xs = [plans[0].new_cols(N, nrhs) for _ in range(R)]
+    Ys = [plans[0].new_cols(N, nrhs) for _ in range(R)]
+    for Xd in Xs:
+        Xd.copy_(torch.from_numpy(RHS).to(dev, torch.float32))
+
+    def step_batched(i):
+        plans[i % R].matmat(Xs[i % R], Ys[i % R], stream=stream)
+
+    def start_kernel_timing():
+        for p in plans:
+            p.timing(True)
+            p.timing_read()
+
+    ms = timed_streams(torch, lambda i, st: plans[i % R].matmat(Xs[i % R], Ys[i % R], stream=st),
+                       args.steps, args.warmup, streams)
+    ms1 = timed(torch, step_batched, args.steps, args.warmup, stream)
+    # kernel durations: single-stream steps again, with HIP events recorded by
+    # the library on `stream` around each kernel (the events add a few
+    # microseconds of gaps per step, so the step times above are taken without)
+    ms_ev = timed(torch, step_batched, args.steps, 0, stream, start_kernel_timing)
+    ph = {"proj_ms": 0.0, "reduce_ms": 0.0, "rows_ms": 0.0, "launches": 0}
+    for p in plans:
+        r = p.timing_read()
+        p.timing(False)
+        for k in ph:
+            ph[k] += r[k]
+    nl = max(ph["launches"], 1)
+    kus = {k: ph[f"{k}_ms"] * 1e3 / nl for k in ("proj", "reduce", "rows")}
+    log(f"batched {storage}: {ms * 1e3:.1f} us per product of {nrhs} rhs with {len(streams)} products in "
+        f"flight, {ms1 * 1e3:.1f} us one at a time ({ms_ev * 1e3:.1f} with kernel events); kernels (us/launch, "
+        f"{ph['launches']} launches): proj {kus['proj']:.1f} reduce {kus['reduce']:.1f} rows {kus['rows']:.1f}")
+    Yb = Ys[0].cpu().numpy().astype(np.float64)
+    relerr_dense = float(np.linalg.norm(Yb[rows] - Yd) / np.linalg.norm(Yd))
+    log(f"batched {storage}: relerr vs dense FP64 ({rows.size} rows) {relerr_dense:.3e}")
+    return {"ms": ms, "ms1": ms1, "ms_ev": ms_ev, "kus": kus, "relerr_dense": relerr_dense,
+            "_plans": (plans, Xs, Ys)}
+
+
+def c4_report(hm, torch, stream) -> dict:
     """64 H-matrices, N log-uniform in [1000, 40000], 1/(1+((x-y)/l)^2),
     l ~ U[0.2, 5], MaxRank in {4, 8, 16}; FP32; one launch (captured in a
     HIP graph) for all members."""
@@ -186,10 +240,16 @@ def c4_report(hm, torch, stream) -> None:
     ms = timed(torch, lambda i: g.launch(stream), 50, 5, stream)
     ms_eager = timed(torch, lambda i: d.matvec(x, y, stream=stream), 50, 5, stream)
     elems = sum(m.stored_elements() for m in members)
+    nbytes = 4 * elems + 8 * d.nrows
     log(f"C4 batch (64 matrices, FP32, 1 rhs each): graph {ms * 1e3:.1f} us, eager {ms_eager * 1e3:.1f} us "
-        f"per launch; {64 / (ms * 1e-3):.0f} H-matvec/s; {(4 * elems + 8 * d.nrows) / (ms * 1e-3) / 1e9:.0f} GB/s")
+        f"per launch; {64 / (ms * 1e-3):.0f} H-matvec/s; {nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
     g.close()
     d.close()
+    return {"workload": "64 H-matrices, N log-uniform in [1000, 40000] (seed 6), 1/(1+((x-y)/l)^2), "
+                        "MaxRank in {4,8,16}, FP32, one right-hand side each, one launch",
+            "total_N": int(d.nrows), "graph_us": ms * 1e3, "eager_us": ms_eager * 1e3,
+            "H-matvec/s": 64 / (ms * 1e-3), "algorithmic_bytes": nbytes,
+            "GB/s": nbytes / (ms * 1e-3) / 1e9, "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def extras_report(hm, torch, stream, plan_b, Xb, Yb) -> dict:
@@ -357,8 +417,10 @@ def main() -> int:
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the measurement, run 3 device copies of this many MB (the plain copy "
                          "kernel the FETCH_SIZE/WRITE_SIZE counters are scaled against)")
-    ap.add_argument("--c4", action="store_true",
-                    help="also time the uneven 64-matrix batch (one launch) and report it on stderr")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the uneven 64-matrix batch (one launch; reported in the line's \"c4\" field)")
+    ap.add_argument("--no-bf16", action="store_true",
+                    help="skip the plain BF16 storage line (the line's \"bf16\" field)")
     ap.add_argument("--extras", action="store_true",
                     help="also time H'X, the FP64 H*y at N=65536 and the FP64 adjoint at N=65536")
     args = ap.parse_args()
@@ -381,54 +443,30 @@ def main() -> int:
         return 2
     streams = [stream] + [torch.cuda.Stream(device=dev) for _ in range(S - 1)]
 
-    # batched low-precision path, rotated replicas
-    t0 = time.time()
-    plans = [hm.DeviceHmat(f, args.storage) for _ in range(args.replicas)]
-    log(f"plans ({args.storage} x{args.replicas}) created in {time.time() - t0:.1f}s; "
-        f"info {plans[0].info()}")
-    Xs = [plans[0].new_cols(N, nrhs) for _ in range(args.replicas)]
-    Ys = [plans[0].new_cols(N, nrhs) for _ in range(args.replicas)]
-    for Xd in Xs:
-        Xd.copy_(torch.from_numpy(RHS).to(dev, torch.float32))
     R = args.replicas
-
-    def step_batched(i):
-        plans[i % R].matmat(Xs[i % R], Ys[i % R], stream=stream)
-
-    def start_kernel_timing():
-        for p in plans:
-            p.timing(True)
-            p.timing_read()
-
-    # throughput: S independent products in flight (replica r on stream r % S)
-    ms = timed_streams(torch, lambda i, st: plans[i % R].matmat(Xs[i % R], Ys[i % R], stream=st),
-                       args.steps, args.warmup, streams)
-    # one product at a time (latency of a product)
-    ms1 = timed(torch, step_batched, args.steps, args.warmup, stream)
-    # kernel durations: single-stream steps again, with HIP events recorded by
-    # the library on `stream` around each of the three kernels (the events add
-    # a few microseconds of gaps per step, so the step times above are taken
-    # without them)
-    ms_ev = timed(torch, step_batched, args.steps, 0, stream, start_kernel_timing)
-    ph = {"proj_ms": 0.0, "reduce_ms": 0.0, "rows_ms": 0.0, "launches": 0}
-    for p in plans:
-        r = p.timing_read()
-        p.timing(False)
-        for k in ph:
-            ph[k] += r[k]
-    nl = max(ph["launches"], 1)
-    kus = {k: ph[f"{k}_ms"] * 1e3 / nl for k in ("proj", "reduce", "rows")}
-    log(f"batched {args.storage}: {ms * 1e3:.1f} us per product of {nrhs} rhs with {S} products in flight, "
-        f"{ms1 * 1e3:.1f} us one at a time ({ms_ev * 1e3:.1f} with kernel events); kernels (us/launch, "
-        f"{ph['launches']} launches): proj {kus['proj']:.1f} reduce {kus['reduce']:.1f} rows {kus['rows']:.1f}")
-
-    # accuracy of the batched result
-    Yb = Ys[0].cpu().numpy().astype(np.float64)
     rows = np.sort(np.random.default_rng(5).choice(N, size=min(args.sample_rows, N), replace=False))
     t0 = time.time()
     Yd = dense_rows_reference(par, c, RHS, rows)
-    relerr_dense = float(np.linalg.norm(Yb[rows] - Yd) / np.linalg.norm(Yd))
-    log(f"dense FP64 sample ({rows.size} rows) in {time.time() - t0:.1f}s; relerr {relerr_dense:.3e}")
+    log(f"dense FP64 sample ({rows.size} rows) in {time.time() - t0:.1f}s")
+    main_run = measure_batched(hm, torch, f, RHS, args.storage, args, streams, rows, Yd)
+    plans, Xs, Ys = main_run.pop("_plans")
+    ms, ms1, ms_ev, kus = main_run["ms"], main_run["ms1"], main_run["ms_ev"], main_run["kus"]
+    relerr_dense = main_run["relerr_dense"]
+    Yb = Ys[0].cpu().numpy().astype(np.float64)
+    # plain BF16 storage (half the factor bytes of the split form), reported
+    # at its own error against the dense FP64 kernel
+    bf16_line = None
+    if args.storage != "bf16" and not args.no_bf16:
+        r16 = measure_batched(hm, torch, f, RHS, "bf16", args, streams, rows, Yd)
+        for p in r16.pop("_plans")[0]:
+            p.close()
+        kb16 = kernel_bytes(f, "bf16", nrhs, 4)
+        g16 = kb16["rows"] / (r16["kus"]["rows"] * 1e-6) / 1e9
+        bf16_line = {"H-matvec/s": nrhs / (r16["ms"] * 1e-3), "us_per_product": r16["ms"] * 1e3,
+                     "single_stream_us": r16["ms1"] * 1e3,
+                     "step_frac": algorithmic_bytes(f, "bf16", nrhs, 4) / (r16["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "kb_strips_us": r16["kus"]["rows"], "kb_strips_GBs": g16, "kb_strips_frac": g16 / HBM_PEAK_GBS,
+                     "relerr_vs_dense_fp64": r16["relerr_dense"]}
 
     # plain FP64 path: one matvec per right-hand side.  The same S-stream
     # schedule: the 32 columns of a step are 32 independent FP64 matvecs,
@@ -465,8 +503,7 @@ def main() -> int:
     pcie_s = time.perf_counter() - t0
     log(f"host-buffer H*X (upload + product + download): {pcie_s * 1e3:.1f} ms")
 
-    if args.c4:
-        c4_report(hm, torch, stream)
+    c4 = None if args.no_c4 else c4_report(hm, torch, stream)
     extras = extras_report(hm, torch, stream, plans, Xs, Ys) if args.extras else None
 
     if args.calibrate_copy_mb > 0:
@@ -539,12 +576,17 @@ def main() -> int:
         "plain_fp64": {"H-matvec/s": plain_mv_s, "us_per_matvec": ms64 * 1e3 / nrhs,
                        "GB/s": bytes_64 / (ms64 * 1e-3 / nrhs) / 1e9,
                        "relerr_vs_dense_fp64": relerr_dense64},
-        "speedup_vs_plain_fp64": value / plain_mv_s,
+        "speedup_vs_per_column_fp64": value / plain_mv_s,
+        "speedup_vs_fp64_matmat": value / (nrhs / (ms64m * 1e-3)),
         "fp64_matmat": {"us_per_product": ms64m * 1e3, "H-matvec/s": nrhs / (ms64m * 1e-3),
                         "relerr_vs_per_column": relerr_m64},
         "host_buffer_product_ms": pcie_s * 1e3,
         "algorithmic_bytes_per_step": bytes_b,
     }
+    if bf16_line is not None:
+        line["bf16"] = bf16_line
+    if c4 is not None:
+        line["c4"] = c4
     if extras is not None:
         line["extras"] = extras
     print(json.dumps(line), flush=True)

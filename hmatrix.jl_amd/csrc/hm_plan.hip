@@ -1069,7 +1069,9 @@ int hm_plan_info(hm_plan_t plan, int64_t *out, int nout) {
                                         p->proj_chunk,
                                         p->row_waves,
                                         p->flags,
-                                        strip_shape_in_use(*p)};
+                                        strip_shape_in_use(*p),
+                                        (p->fwd.bat.ready &&
+                                         ((p->flags & HM_NO_TRANSPOSE) || p->bwd.bat.ready)) ? 1 : 0};
     for (int i = 0; i < nout && i < HM_INFO_NFIELDS; ++i) out[i] = v[i];
     return HM_OK;
 }

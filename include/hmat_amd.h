@@ -68,6 +68,12 @@ enum hm_info_field {
     HM_INFO_ROW_WAVES,       /* hm_config.row_waves in effect                    */
     HM_INFO_FLAGS,           /* hm_config.flags                                  */
     HM_INFO_STRIP_SHAPE,     /* enum hm_strip_shape of the H X row pass in use    */
+    HM_INFO_BATCHED,         /* 1: the multi-right-hand-side MFMA layout was built
+                                for H (and H' unless HM_NO_TRANSPOSE); 0: it was
+                                not (FP64 storage, HM_NO_BATCHED, fewer than 8
+                                columns, or slice offsets past 2^32), and
+                                hm_matmat runs panels of 8 columns on the
+                                single-right-hand-side layout instead          */
     HM_INFO_NFIELDS
 };
 
