@@ -41,6 +41,7 @@ SIGNATURES = {
     "hm_plan_destroy": (ctypes.c_int, [_P]),
     "hm_plan_info": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.c_int]),
     "hm_matvec": (ctypes.c_int, [_P, _P, _P, _D, _D, _P]),
+    "hm_matvec_static": (ctypes.c_int, [_P, _P, _P, _D, _D, _P]),
     "hm_matmat": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _D, _D, _P]),
     "hm_matvec_t": (ctypes.c_int, [_P, _P, _P, _D, _D, _P]),
     "hm_matmat_t": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _D, _D, _P]),

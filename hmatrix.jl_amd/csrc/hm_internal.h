@@ -247,9 +247,11 @@ struct Plan {
 
 // Launchers (hm_kernels.hip).  Vectors are double for HM_F64, float otherwise.
 // ev (may be NULL): three events recorded after the projection, the
-// reduction and the row pass.
+// reduction and the row pass.  scratch (may be NULL): device buffer of at
+// least (d.tLen + d.partLen) accumulator elements used instead of d.t.
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y,
-                         double alpha, double beta, hipStream_t s, hipEvent_t *ev = nullptr);
+                         double alpha, double beta, hipStream_t s, hipEvent_t *ev = nullptr,
+                         void *scratch = nullptr);
 // Adjoint: gradients from res, x and the t vectors the two products left in
 // fwd.t (t = B' x) and bwd.t (s = A' res); optional loss into p.lossbuf.
 hipError_t launch_adjoint_grads(const Plan &p, const void *x, const void *res, void *grad,

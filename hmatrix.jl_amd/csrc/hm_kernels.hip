@@ -564,11 +564,11 @@ hipError_t launch_adjoint_grads(const Plan &p, const void *x, const void *res, v
 
 template <class St>
 static hipError_t run(const Plan &p, const Direction &d, const void *xv, void *yv, double alpha,
-                      double beta, hipStream_t s, hipEvent_t *ev) {
+                      double beta, hipStream_t s, hipEvent_t *ev, void *scratch) {
     using A = typename St::acc;
     const A *x = static_cast<const A *>(xv);
     A *y = static_cast<A *>(yv);
-    A *t = static_cast<A *>(d.t);
+    A *t = static_cast<A *>(scratch ? scratch : d.t);
     hipError_t e = hipSuccess;
     if (d.nproj > 0) {
         // The scratch buffer is [t (tLen) | partials (partLen)]: items that
@@ -660,12 +660,12 @@ hipError_t launch_matvec_m(const Plan &p, const Direction &d, int ncol, const vo
 }
 
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y, double alpha,
-                         double beta, hipStream_t s, hipEvent_t *ev) {
+                         double beta, hipStream_t s, hipEvent_t *ev, void *scratch) {
     switch (p.storage) {
-        case 0: return run<StF64>(p, d, x, y, alpha, beta, s, ev);
-        case 1: return run<StF32>(p, d, x, y, alpha, beta, s, ev);
-        case 2: return run<StBF16>(p, d, x, y, alpha, beta, s, ev);
-        case 3: return run<StBF16x2>(p, d, x, y, alpha, beta, s, ev);
+        case 0: return run<StF64>(p, d, x, y, alpha, beta, s, ev, scratch);
+        case 1: return run<StF32>(p, d, x, y, alpha, beta, s, ev, scratch);
+        case 2: return run<StBF16>(p, d, x, y, alpha, beta, s, ev, scratch);
+        case 3: return run<StBF16x2>(p, d, x, y, alpha, beta, s, ev, scratch);
         default: return hipErrorInvalidValue;
     }
 }

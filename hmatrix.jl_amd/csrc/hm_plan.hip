@@ -20,6 +20,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 
@@ -743,8 +744,11 @@ int check_vectors(const hm::Plan *p, const void *x, const void *y) {
     return HM_OK;
 }
 
+// scratch (may be NULL, single right-hand side only): device buffer used
+// instead of the direction's own t / partial-sum buffer.
 int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void *X, int64_t ldx,
-                  void *Y, int64_t ldy, double alpha, double beta, void *stream) {
+                  void *Y, int64_t ldy, double alpha, double beta, void *stream,
+                  void *scratch = nullptr) {
     if (nrhs < 0) return fail(HM_EINVAL, "nrhs < 0");
     if (nrhs > 0 && (ldx < d.ncols || ldy < d.nrows || ldx < 1 || ldy < 1))
         return fail(HM_EINVAL, "leading dimension smaller than the vector length");
@@ -764,8 +768,9 @@ int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void 
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t es = acc_size(p->storage);
     // Phase timing records one set of four events per launch sequence; none
-    // while the stream is being captured into a graph.
-    bool timed = p->timing.on;
+    // while the stream is being captured into a graph, nor for products on
+    // the static scratch.
+    bool timed = p->timing.on && !scratch;
     if (timed) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) timed = false;
@@ -789,8 +794,9 @@ int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void 
         }
         return HM_OK;
     }
+    if (scratch && nrhs != 1) return fail(HM_EINVAL, "the static scratch serves one right-hand side");
     if (nrhs == 1) {
-        hipError_t e = hm::launch_matvec(*p, d, X, Y, alpha, beta, s, record());
+        hipError_t e = hm::launch_matvec(*p, d, X, Y, alpha, beta, s, record(), scratch);
         if (e != hipSuccess)
             return fail(HM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
         return HM_OK;
@@ -1120,6 +1126,63 @@ int hm_matvec(hm_plan_t plan, const void *x, void *y, double alpha, double beta,
     if (rc) return rc;
     return run_direction(p, p->fwd, 1, x, p->ncols > 0 ? p->ncols : 1, y,
                          p->nrows > 0 ? p->nrows : 1, alpha, beta, stream);
+}
+
+}  // extern "C"
+
+namespace {
+// Scratch of hm_matvec_static: the t vector and chunk partials of one
+// single-right-hand-side product.  Every device that runs the library holds
+// its own copy of this array until the process ends.  Host code never uses
+// the name `scratch` as an address: static_scratch() takes the device
+// address once per device with hipGetSymbolAddress and every launch passes
+// that pointer.
+__device__ __align__(256) unsigned char scratch[HM_STATIC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on device dev (current on entry).
+int static_scratch(int dev, void **out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(HM_EUNSUPPORTED, "device index past the static scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(HM_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return HM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hm_matvec_static(hm_plan_t plan, const void *x, void *y, double alpha, double beta, void *stream) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    int rc = check_vectors(p, x, y);
+    if (rc) return rc;
+    const hm::Direction &d = p->fwd;
+    const int64_t need = (d.tLen + d.partLen) * static_cast<int64_t>(acc_size(p->storage));
+    if (need > static_cast<int64_t>(HM_STATIC_SCRATCH_BYTES)) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "hm_matvec_static: this plan needs %lld bytes of scratch, more than the static "
+                      "maximum of %lld (HM_STATIC_SCRATCH_BYTES); use hm_matvec",
+                      (long long)need, (long long)HM_STATIC_SCRATCH_BYTES);
+        return fail(HM_EUNSUPPORTED, msg);
+    }
+    void *ptr = nullptr;
+    {
+        DeviceGuard g(p->device);
+        rc = static_scratch(p->device, &ptr);
+        if (rc) return rc;
+    }
+    return run_direction(p, d, 1, x, p->ncols > 0 ? p->ncols : 1, y, p->nrows > 0 ? p->nrows : 1, alpha,
+                         beta, stream, ptr);
 }
 
 int hm_matmat(hm_plan_t plan, int64_t nrhs, const void *X, int64_t ldx, void *Y, int64_t ldy,

@@ -174,6 +174,23 @@ int hm_matvec(hm_plan_t plan, const void *x, void *y,
               double alpha, double beta, void *stream);
 
 /*
+ * hm_matvec that allocates nothing and uses no scratch memory of the plan.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, result and kernels as
+ * hm_matvec; the projection's t vector and partial sums live in a
+ * library-wide static device array of HM_STATIC_SCRATCH_BYTES per GPU
+ * (held until the process ends) instead of the plan's buffer.  A plan whose
+ * (sum of ranks + chunk partials) * element size exceeds that maximum is
+ * rejected with HM_EUNSUPPORTED.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (two streams, or two threads) race
+ * and must be serialised by the caller.  Not recorded by hm_plan_timing.
+ */
+#define HM_STATIC_SCRATCH_BYTES (4u << 20)
+int hm_matvec_static(hm_plan_t plan, const void *x, void *y,
+                     double alpha, double beta, void *stream);
+
+/*
  * Y = beta*Y + alpha*H*X for nrhs right-hand sides, column-major X
  * (ncols x nrhs, leading dimension ldx) and Y (nrows x nrhs, ldy).  FP32,
  * BF16 and split-BF16 plans run panels of 32 columns on the batched MFMA
