@@ -14,18 +14,21 @@
 //   kb_reduce sums the per-super-tile partials of leaves wider than one
 //             super-tile, in a fixed order (deterministic, no atomics),
 //             largest leaves first.
-//   kb_strips row owner: one wavefront per 16-row strip, 8 or 16 per
-//             workgroup (the plan's row shape).  A strip is a list of
-//             K-steps of 32 columns: each
-//             dense leaf crossing it (A = its rows, B = X rows of its
-//             columns) and a panel holding the A columns of every low-rank
-//             leaf crossing it (B = the matching t rows).  A operands are
-//             pre-packed in MFMA fragment order (zero rows and octets not
-//             stored) and loaded straight into registers, K-steps in
-//             batches; X rows come from a workgroup window staged in LDS.
-//             Each element of Y is written once.  Workgroups are remapped
-//             so consecutive strips run on the same XCD and share X rows in
-//             its L2.
+//   kb_rows   row owner: a workgroup owns a band of strips of 16 rows (8
+//             or 16 wavefronts, two strips each; the plan's row shape) and
+//             stages the band's X window in LDS once, split into BF16 hi/lo
+//             in the B-operand order.  A strip is a list of K-steps of 32
+//             columns: each dense leaf crossing it (A = its rows, B = X
+//             rows of its columns) and a panel holding the A columns of
+//             every low-rank leaf crossing it (B = the matching t rows).
+//             A operands are pre-packed in MFMA fragment order (zero rows
+//             and octets not stored); each wavefront walks the records of
+//             its strips as one software-pipelined stream (the A fragments
+//             of K-step k + 2 in flight during the MFMAs of K-step k, the
+//             next record's panel and first K-steps issued before the
+//             current record's last MFMAs).  Each element of Y is written
+//             once.  Workgroups are remapped so consecutive bands run on
+//             the same XCD and share X rows in its L2.
 // Storage modes: HM_BF16X2 (dense leaves as hi+lo BF16 planes, 3 MFMAs per
 // product term: hi*hi + hi*lo + lo*hi; low-rank factors FP32), HM_F32 (dense
 // leaves FP32, split in registers, same 3 MFMAs), HM_BF16 (dense leaves and
@@ -42,12 +45,6 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef short short8 __attribute__((ext_vector_type(8)));
 
 enum : int { MODE_F32 = 1, MODE_BF16 = 2, MODE_BF16X2 = 3 };
-
-// K-steps a row-pass wavefront loads before their MFMAs.
-#ifndef HM_SKC
-#define HM_SKC 2
-#endif
-constexpr int SKC = HM_SKC;
 
 __device__ __forceinline__ uint32_t f2bf_bits(float f) {
     uint32_t u = __float_as_uint(f);
@@ -170,8 +167,9 @@ __global__ __launch_bounds__(256) void kb_proj(int64_t gps, const uint4 *__restr
     const int g = l >> 4;
     const uint4 *gdesc = pcol + static_cast<int64_t>(blockIdx.x) * gps * 16;
     // every wavefront loads a descriptor (the last group's when it has none)
-    // and empties its K-step range when the index is past the groups
-    uint4 cd = gdesc[min(wave, static_cast<int>(gps) - 1) * 16 + r16];
+    // and empties its K-step range when the index is past the groups; a
+    // plan without low-rank columns (gps == 0) has no descriptors
+    uint4 cd = gps > 0 ? gdesc[min(wave, static_cast<int>(gps) - 1) * 16 + r16] : uint4{0u, 0u, 0xffffffffu, 0u};
     if (wave >= gps) cd = uint4{0u, cd.y & 0xffff0000u, 0xffffffffu, 0u};
     // Stage X[col0:col0+rows] in (column, 4-row quad) units: thread t reads
     // quad t % 64 of columns t / 64 + 4u, a wavefront 1 KB of one column per
@@ -332,16 +330,6 @@ __global__ __launch_bounds__(256) void kb_reduce(const BRed *__restrict__ red, i
 }
 
 // ---------------------------------------------------------------------------
-// row pass: one wavefront per strip of SROWS rows x NB right-hand sides,
-// SW strips per workgroup.  Per K-step a lane loads its 8 A values of
-// strip row r16 (octet g) and, for each right-hand-side block h, the 8 B
-// values of k-rows [8g, 8g + 8) in column 16h + r16: X rows from the
-// workgroup's X window in LDS (or straight from X when the window is wider
-// than XSM), panel rows gathered from t.  KC K-steps are loaded together
-// before their MFMAs.  The strip record and the first batch's A fragments
-// and t-row indices are loaded before the window's barrier, in flight
-// together with the window's X loads.
-
 // lane's 8 A values of one K-step as BF16 hi / lo vectors
 template <int MODE>
 __device__ __forceinline__ void load_a(const void *frag, const float *zeros, bool on, uint32_t afrag,
@@ -362,81 +350,136 @@ __device__ __forceinline__ void load_a(const void *frag, const float *zeros, boo
     }
 }
 
-template <int MODE, int KC, int SW, int XSM>
-__global__ __launch_bounds__(64 * SW) void kb_strips(int64_t nstrips, int64_t wg_per_xcd,
-                                                 const uint32_t *__restrict__ soff,
-                                                 const uint32_t *__restrict__ recs,
-                                                 const int32_t *__restrict__ wgx,
-                                                 const void *__restrict__ frag,
-                                                 const float *__restrict__ zeros,
-                                                 const float *__restrict__ T,
-                                                 const float *__restrict__ X, int64_t ldx, int64_t ncols,
-                                                 float *__restrict__ Y, int64_t ldy,
-                                                 int ncol, float alpha, float beta, int beta_zero) {
-    // X window of the workgroup as FP32 [column of X][row], then 8 zeros
-    // that lanes without data read
-    constexpr int XS_LD = XSM + 4;
-    static_assert((XS_LD / 4) % 2 == 1, "odd 16-byte stride");
-    __shared__ float xs[NB * XS_LD + 8];
-    // Hardware places workgroup b on XCD b % 8; give each XCD a contiguous
-    // range of strips so neighbouring strips share X rows in that L2.
-    const int64_t wg = static_cast<int64_t>(blockIdx.x % 8) * wg_per_xcd + blockIdx.x / 8;
-    if (wg * SW >= nstrips) return;           // whole workgroup
+// ---------------------------------------------------------------------------
+// Row pass.  Workgroup b (remapped so that consecutive bands run on one XCD
+// and share X rows in its L2) owns a band of NW * SPW strips; wavefront w
+// takes strips w, w + NW, ... of the band.  The band's X window
+// [cmin, cmin + W) (every dense piece of the band, W <= XSM rows) is staged
+// once in LDS, split into BF16 hi/lo planes in the B-operand order; a band
+// whose pieces span more rows (W = 0) reads X straight from memory.  Each
+// wavefront then walks the records of its strips as one stream.  A record
+// has nd >= 0 dense K-steps and at most one panel K-step; the wavefront
+// treats it as max(nd, 1) dense K-steps (padded to an even count) and one
+// panel K-step, the missing ones read from the zero buffer, so that every
+// record issues the same load sequence and the compiler's wait counts stay
+// exact:
+//   * dense K-steps run in pairs through two A-operand sets: the A
+//     fragments of K-step k + 1 are loaded before the MFMAs of K-step k,
+//     whose B operand is read from the LDS window;
+//   * at the last dense K-step, the registers of the record after next, the
+//     next record's panel K-step (A fragments, 16 gathered t values, row
+//     indices taken from the record registers by ds_bpermute) and its first
+//     dense A fragments are issued first, then the current record's last
+//     dense and panel MFMAs run and its Y rows are written (unless the strip
+//     continues in a chained record).
+
+__device__ __forceinline__ uint32_t rword(uint32_t h0, uint32_t h1, int i) {
+    const uint32_t a = __builtin_amdgcn_readlane(h0, i & 63);
+    const uint32_t b = __builtin_amdgcn_readlane(h1, i & 63);
+    return i < 64 ? a : b;
+}
+
+// A operand of one K-step, and a panel K-step's operands (A, then 16 t
+// values: rows 8g..8g+7 of columns r16 and 16 + r16)
+struct AOp {
+    bf16x8 ah, al;
+};
+struct POp {
+    bf16x8 ah, al;
+    float t[16];
+};
+
+template <int MODE, int NW, int SPW, int XSM, bool STAGED>
+__global__ __launch_bounds__(64 * NW, 4) void kb_rows(int64_t nstrips, int64_t wg_per_xcd,
+                                                   const uint32_t *__restrict__ soff,
+                                                   const uint32_t *__restrict__ recs,
+                                                   const int32_t *__restrict__ wgx,
+                                                   const uint32_t *__restrict__ ulist,
+                                                   const void *__restrict__ frag,
+                                                   const float *__restrict__ zeros,
+                                                   const float *__restrict__ T,
+                                                   const float *__restrict__ X, int64_t ldx, int64_t ncols,
+                                                   float *__restrict__ Y, int64_t ldy, int ncol, float alpha,
+                                                   float beta, int beta_zero) {
+    // window: [hi, lo plane][column of X][octet] vectors of 8 BF16 (rows
+    // 8 octet .. of the window); odd octet stride: the B reads of 16
+    // consecutive lanes (16 columns) and the staging writes (consecutive
+    // rows of one column) fall on distinct banks
+    constexpr int NOCT = XSM / 8;
+    constexpr int OLD = NOCT + 1;
+    static_assert(OLD % 2 == 1, "odd octet stride");
+    __shared__ uint4 xs[STAGED ? 2 : 1][STAGED ? NB : 1][STAGED ? OLD : 1];
+    // staged instance: band b%8 * per + b/8; the other instance: band list[b]
+    const int64_t wg = STAGED ? static_cast<int64_t>(blockIdx.x % 8) * wg_per_xcd + blockIdx.x / 8
+                              : static_cast<int64_t>(ulist[blockIdx.x]);
+    constexpr int BAND = NW * SPW;
+    if (wg * BAND >= nstrips) return;          // whole workgroup
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    const int64_t s = wg * SW + wave;
     const int l = threadIdx.x & 63;
     const int r16 = l & 15;
     const int g = l >> 4;
     const int cmin = wgx[2 * wg], W = wgx[2 * wg + 1];
-    const float *xzero = xs + NB * XS_LD;
+    if ((W > 0) != STAGED) return;             // the band is the other instance's
+    const int64_t first = wg * BAND + wave;    // strips first + NW j
+    const int nrun = first < nstrips ? static_cast<int>(min<int64_t>(SPW, (nstrips - first + NW - 1) / NW)) : 0;
+    const uint32_t sv = soff[first + static_cast<int64_t>(NW) * min(l, max(nrun - 1, 0))];
+    constexpr int AU = MODE == MODE_F32 ? 2 : 1;     // 16-byte units per lane and plane
+    constexpr uint32_t NONE = 0xffffffffu;
 
-    const bool has = s < nstrips;
-    constexpr int NQ = XSM / 4;                    // quads per column slot
-    constexpr int U = NB * NQ / (64 * SW);         // units per thread
-    static_assert(U * 64 * SW == NB * NQ, "whole units per thread");
+    // Stage the window: thread t takes (column, 4-row quad) units along the
+    // rows of one column, loads all of them, then splits them into LDS.
+    constexpr int NQ = XSM / 4;
+    constexpr int U = (NB * NQ + 64 * NW - 1) / (64 * NW);     // units per thread (the last partly)
+    // rows of unit u of this thread that lie in the window and in X (0..4)
+    auto unit_rows = [&](int u) __attribute__((always_inline)) {
+        const int p = threadIdx.x + 64 * NW * u;
+        const int c = p / NQ, q = p % NQ;
+        const int64_t row = static_cast<int64_t>(cmin) + 4 * q;
+        return (p < NB * NQ && c < ncol && 4 * q < W) ? static_cast<int>(min<int64_t>(4, ncols - row)) : 0;
+    };
+    f4u xv[U];
+    if constexpr (STAGED) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = threadIdx.x + 64 * NW * u;
+            const int c = p / NQ, q = p % NQ;
+            const float *src = unit_rows(u) == 4 ? X + static_cast<int64_t>(c) * ldx + cmin + 4 * q : zeros;
+            xv[u] = *reinterpret_cast<const f4u *>(src);
+        }
+    }
 
-    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    const bool rok0 = r16 < ncol, rok1 = 16 + r16 < ncol;
-    const float *xc0 = X + static_cast<int64_t>(r16) * ldx;
-    const float *xc1 = X + static_cast<int64_t>(16 + r16) * ldx;
-    int row0 = 0, R = 0;
-    // Strip record: its first 128 words in two vector registers (lane l holds
-    // words l and 64 + l; the record arena is padded so these loads stay in
-    // bounds), read with readlane -- one memory round trip for the header
-    // and every K-step descriptor.  A wavefront without a strip reads strip
-    // 0's record (valid addresses, results unused).
-    uint32_t rec = soff[has ? s : 0];
-    const uint32_t *rw = recs + rec;
-    uint32_t hv0 = rw[l], hv1 = rw[64 + l];
-    int nd = 0, np = 0;
-    uint32_t pidx = 0;
-    auto word = [&](int i) __attribute__((always_inline)) -> uint32_t {
-        const uint32_t a = __builtin_amdgcn_readlane(hv0, i & 63);
-        const uint32_t b = __builtin_amdgcn_readlane(hv1, i & 63);
-        return i < 64 ? a : b;
-    };
-    auto decode = [&]() __attribute__((always_inline)) {
-        row0 = static_cast<int>(word(0));
-        R = static_cast<int>(word(1));
-        nd = static_cast<int>(word(2));
-        np = static_cast<int>(word(3));
-        pidx = word(4);
-    };
-    auto desc = [&](int k) __attribute__((always_inline)) {
+    auto issue_a = [&](uint32_t h0, uint32_t h1, int k, bool on, AOp &o) __attribute__((always_inline)) {
         const int i = SREC_HDR + 4 * k;
-        return uint4{word(i), word(i + 1), word(i + 2), 0u};
+        const uint32_t afrag = rword(h0, h1, i), info = rword(h0, h1, i + 2);
+        const int roff = info & 15, rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
+        load_a<MODE>(frag, zeros, on && g < nOct && r16 >= roff && r16 < roff + rows, afrag,
+                     (g * rows + r16 - roff) * AU, nOct * rows, o.ah, o.al);
     };
-    // MFMAs of one K-step: 8 B values per lane and block (hi/lo split here)
-    auto kstep = [&](const bf16x8 &ah, const bf16x8 &al, const f4u (&b)[4]) __attribute__((always_inline)) {
-        bf16x8 xh0, xl0, xh1, xl1;
-        {
-            const float v[8] = {b[0][0], b[0][1], b[0][2], b[0][3], b[1][0], b[1][1], b[1][2], b[1][3]};
-            split8hw(v, xh0, xl0);
+    auto issue_panel = [&](uint32_t h0, uint32_t h1, int k, bool on, POp &o) __attribute__((always_inline)) {
+        const int i = SREC_HDR + 4 * k;
+        const uint32_t afrag = rword(h0, h1, i), src = rword(h0, h1, i + 1), info = rword(h0, h1, i + 2);
+        const int rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
+        load_a<MODE>(frag, zeros, on && g < nOct && r16 < rows, afrag, (g * rows + r16) * AU, nOct * rows, o.ah,
+                     o.al);
+        uint32_t ix[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int wi = (static_cast<int>(src) + 8 * g + e) & 127;   // record word holding the t-row index
+            const int a = (wi & 63) << 2;
+            const uint32_t i0 = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(a, static_cast<int>(h0)));
+            const uint32_t i1 = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(a, static_cast<int>(h1)));
+            ix[e] = wi < 64 ? i0 : i1;
         }
-        {
-            const float v[8] = {b[2][0], b[2][1], b[2][2], b[2][3], b[3][0], b[3][1], b[3][2], b[3][3]};
-            split8hw(v, xh1, xl1);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float *tr = on ? T + static_cast<int64_t>(ix[e]) * NB + r16 : zeros;
+            o.t[e] = tr[0];
+            o.t[8 + e] = tr[16];
         }
+    };
+    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    auto mfmas = [&](const bf16x8 &ah, const bf16x8 &al, const bf16x8 &xh0, const bf16x8 &xl0,
+                     const bf16x8 &xh1, const bf16x8 &xl1) __attribute__((always_inline)) {
         acc0 = mfma(ah, xh0, acc0);
         acc0 = mfma(ah, xl0, acc0);
         acc1 = mfma(ah, xh1, acc1);
@@ -446,132 +489,61 @@ __global__ __launch_bounds__(64 * SW) void kb_strips(int64_t nstrips, int64_t wg
             acc1 = mfma(al, xh1, acc1);
         }
     };
-    // Panel K-step q: A fragments (on == false: the zero buffer); the t rows
-    // of this lane's octet are indices [8g, 8g + 8) of a 32-word index block
-    // of the record.  Every strip has at least one index block (zero rows
-    // when it has no panel), so the loads issued never depend on the strip.
-    auto load_panel_a = [&](int q, bool on, bf16x8 &ah, bf16x8 &al) __attribute__((always_inline)) {
-        const uint4 kd = desc(nd + (on ? q : 0));
-        const uint32_t afrag = kd.x, info = kd.z;
-        const int rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
-        load_a<MODE>(frag, zeros, on && g < nOct && r16 < rows, afrag,
-                     (g * rows + r16) * (MODE == MODE_F32 ? 2 : 1), nOct * rows, ah, al);
-    };
-    auto gather_t = [&](const uint4 &q0, const uint4 &q1, f4u (&b)[4]) __attribute__((always_inline)) {
-        const uint32_t ix[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float *tr = T + static_cast<int64_t>(ix[e]) * NB + r16;
-            b[e >> 2][e & 3] = tr[0];
-            b[2 + (e >> 2)][e & 3] = tr[16];
-        }
-    };
-    // A operands of dense K-steps [k0, k0 + KC); K-steps past the strip's end
-    // load the zero buffer, and their MFMAs run anyway (a branch around them
-    // would let the compiler move their loads behind the first wait)
-    auto load_dense_a = [&](int k0, bf16x8 (&ah)[KC], bf16x8 (&al)[KC]) __attribute__((always_inline)) {
-#pragma unroll
-        for (int u = 0; u < KC; ++u) {
-            const uint4 kd = desc(max(0, min(k0 + u, nd - 1)));
-            const bool live = k0 + u < nd;
-            const uint32_t afrag = kd.x, info = kd.z;
-            const int roff = info & 15, rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
-            const bool aon = live && g < nOct && r16 >= roff && r16 < roff + rows;
-            load_a<MODE>(frag, zeros, aon, afrag, (g * rows + r16 - roff) * (MODE == MODE_F32 ? 2 : 1),
-                         nOct * rows, ah[u], al[u]);
-        }
-    };
-    // B operands of dense K-steps [k0, k0 + KC): X rows from the LDS window
-    // (16-byte aligned reads) or straight from X
-    auto load_dense_b = [&](int k0, auto staged, f4u (&b)[KC][4]) __attribute__((always_inline)) {
-        constexpr bool STAGED = decltype(staged)::value;
-#pragma unroll
-        for (int u = 0; u < KC; ++u) {
-            const uint4 kd = desc(max(0, min(k0 + u, nd - 1)));
-            const bool live = k0 + u < nd;
-            const uint32_t src = kd.y, info = kd.z;
-            const int nOct = (info >> 9) & 7;
-            const bool on = live && g < nOct;
-            if constexpr (STAGED) {
-                const int off = static_cast<int>(src) - cmin + 8 * g;
-                const float *p0 = on ? xs + r16 * XS_LD + off : xzero;
-                const float *p1 = on ? xs + (16 + r16) * XS_LD + off : xzero;
-                const float4 v0 = reinterpret_cast<const float4 *>(p0)[0];
-                const float4 v1 = reinterpret_cast<const float4 *>(p0)[1];
-                const float4 v2 = reinterpret_cast<const float4 *>(p1)[0];
-                const float4 v3 = reinterpret_cast<const float4 *>(p1)[1];
-                b[u][0] = f4u{v0.x, v0.y, v0.z, v0.w};
-                b[u][1] = f4u{v1.x, v1.y, v1.z, v1.w};
-                b[u][2] = f4u{v2.x, v2.y, v2.z, v2.w};
-                b[u][3] = f4u{v3.x, v3.y, v3.z, v3.w};
-            } else {
-                const int64_t off = static_cast<int64_t>(src) + 8 * g;
-                const float *p0 = on && rok0 ? xc0 + off : zeros;
-                const float *p1 = on && rok1 ? xc1 + off : zeros;
-                b[u][0] = reinterpret_cast<const f4u *>(p0)[0];
-                b[u][1] = reinterpret_cast<const f4u *>(p0)[1];
-                b[u][2] = reinterpret_cast<const f4u *>(p1)[0];
-                b[u][3] = reinterpret_cast<const f4u *>(p1)[1];
+    const bf16x8 zero8 = __builtin_bit_cast(bf16x8, short8{0, 0, 0, 0, 0, 0, 0, 0});
+    // dense K-step k of record (h0, h1) with its A operand; B from the LDS
+    // window, or from X when the band is not staged
+    auto compute_dense = [&](uint32_t h0, uint32_t h1, int k, bool on, const AOp &a,
+                             auto staged) __attribute__((always_inline)) {
+        constexpr bool FROM_LDS = decltype(staged)::value;
+        const int i = SREC_HDR + 4 * k;
+        const uint32_t src = rword(h0, h1, i + 1), info = rword(h0, h1, i + 2);
+        if (!on) return;                          // a padding step: nothing to add
+        const int nOct = (info >> 9) & 7;
+        const bool bon = g < nOct;
+        bf16x8 xh0 = zero8, xl0 = zero8, xh1 = zero8, xl1 = zero8;
+        if constexpr (FROM_LDS) {
+            const int o = ((static_cast<int>(src) - cmin) >> 3) + g;
+            if (bon) {
+                xh0 = __builtin_bit_cast(bf16x8, xs[0][r16][o]);
+                xl0 = __builtin_bit_cast(bf16x8, xs[1][r16][o]);
+                xh1 = __builtin_bit_cast(bf16x8, xs[0][16 + r16][o]);
+                xl1 = __builtin_bit_cast(bf16x8, xs[1][16 + r16][o]);
             }
-        }
-    };
-    auto batch = [&](int k0, auto staged) __attribute__((always_inline)) {
-        bf16x8 ah[KC], al[KC];
-        f4u b[KC][4];
-        load_dense_a(k0, ah, al);
-        load_dense_b(k0, staged, b);
-        __builtin_amdgcn_sched_barrier(0);
+        } else {
+            float v0[8], v1[8];
+            const int64_t row = static_cast<int64_t>(src) + 8 * g;
 #pragma unroll
-        for (int u = 0; u < KC; ++u) kstep(ah[u], al[u], b[u]);
-    };
-    // rest of a record after its first batch and first panel K-step
-    auto rest = [&](auto staged) __attribute__((always_inline)) {
-        for (int k0 = KC; k0 < nd; k0 += KC) batch(k0, staged);
-        for (int q = 1; q < np; ++q) {
-            bf16x8 ah, al;
-            f4u b[4];
-            load_panel_a(q, true, ah, al);
-            const uint4 *ip = reinterpret_cast<const uint4 *>(rw + desc(nd + q).y + 8 * g);
-            gather_t(ip[0], ip[1], b);
-            kstep(ah, al, b);
-        }
-    };
-    // first batch of a record: loads (issued early), then B reads and MFMAs
-    auto first_loads = [&](uint4 &q0, uint4 &q1, bf16x8 &pah, bf16x8 &pal, bf16x8 (&ah)[KC],
-                           bf16x8 (&al)[KC]) __attribute__((always_inline)) {
-        const uint4 *ip = reinterpret_cast<const uint4 *>(rw + pidx + 8 * g);
-        q0 = ip[0];
-        q1 = ip[1];
-        load_panel_a(0, np > 0, pah, pal);
-        load_dense_a(0, ah, al);
-    };
-    auto first_compute = [&](auto staged, const uint4 &q0, const uint4 &q1, const bf16x8 &pah,
-                             const bf16x8 &pal, const bf16x8 (&ah)[KC], const bf16x8 (&al)[KC]) __attribute__((always_inline)) {
-        f4u b[KC][4];
-        load_dense_b(0, staged, b);
-        f4u pb[4];
-        __builtin_amdgcn_sched_barrier(0);
-        gather_t(q0, q1, pb);
-        __builtin_amdgcn_sched_barrier(0);
+            for (int h = 0; h < 2; ++h) {
+                float *v = h == 0 ? v0 : v1;
+                const int c = 16 * h + r16;
+                const float *xp = X + static_cast<int64_t>(c) * ldx + row;
+                if (bon && c < ncol && row + 8 <= ncols) {
+                    const f4u p0 = reinterpret_cast<const f4u *>(xp)[0];
+                    const f4u p1 = reinterpret_cast<const f4u *>(xp)[1];
 #pragma unroll
-        for (int u = 0; u < KC; ++u) kstep(ah[u], al[u], b[u]);
-        kstep(pah, pal, pb);
-    };
-    auto finish = [&](auto staged) __attribute__((always_inline)) {
-        rest(staged);
-        rec = word(5);
-        while (rec != 0u) {                        // chained records
-            rw = recs + rec;
-            hv0 = rw[l];
-            hv1 = rw[64 + l];
-            decode();
-            uint4 q0, q1;
-            bf16x8 pah, pal, ah[KC], al[KC];
-            first_loads(q0, q1, pah, pal, ah, al);
-            first_compute(staged, q0, q1, pah, pal, ah, al);
-            rest(staged);
-            rec = word(5);
+                    for (int e = 0; e < 4; ++e) {
+                        v[e] = p0[e];
+                        v[4 + e] = p1[e];
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) v[e] = (bon && c < ncol && row + e < ncols) ? xp[e] : 0.0f;
+                }
+            }
+            split8hw(v0, xh0, xl0);
+            split8hw(v1, xh1, xl1);
         }
+        mfmas(a.ah, a.al, xh0, xl0, xh1, xl1);
+    };
+    auto compute_panel = [&](const POp &o) __attribute__((always_inline)) {
+        const float v0[8] = {o.t[0], o.t[1], o.t[2], o.t[3], o.t[4], o.t[5], o.t[6], o.t[7]};
+        const float v1[8] = {o.t[8], o.t[9], o.t[10], o.t[11], o.t[12], o.t[13], o.t[14], o.t[15]};
+        bf16x8 xh0, xl0, xh1, xl1;
+        split8hw(v0, xh0, xl0);
+        split8hw(v1, xh1, xl1);
+        mfmas(o.ah, o.al, xh0, xl0, xh1, xl1);
+    };
+    auto store = [&](int row0, int R) __attribute__((always_inline)) {
         // D layout: lane (g, r16) holds strip rows 4g .. 4g+3 of column 16h + r16
         const int r = 4 * g;
         if (r < R) {
@@ -592,71 +564,125 @@ __global__ __launch_bounds__(64 * SW) void kb_strips(int64_t nstrips, int64_t wg
                 }
             }
         }
+        acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
+        acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+    };
+    // record after (h0, h1) of strip position at: the chain's continuation,
+    // else the next strip's first record (NONE past the wavefront's strips)
+    auto next_of = [&](uint32_t h0, uint32_t h1, int at, int &nat) __attribute__((always_inline)) -> uint32_t {
+        const uint32_t ch = rword(h0, h1, 5);
+        if (ch != 0u) {
+            nat = at;
+            return ch;
+        }
+        nat = at + 1;
+        return at + 1 < nrun ? __builtin_amdgcn_readlane(sv, (at + 1) & 63) : NONE;
+    };
+    auto load_rec = [&](uint32_t off, uint32_t &h0, uint32_t &h1) __attribute__((always_inline)) {
+        const uint32_t *rw = recs + (off == NONE ? 0u : off);     // record 0 stands in for none
+        h0 = rw[l];
+        h1 = rw[64 + l];
     };
 
-    if (W > 0) {
-        // The record loads (issued above), the X window loads, then -- once
-        // the record has arrived -- the first batch's A and index loads: all
-        // in flight together; the LDS stores wait only for the X loads.
-        f4u v[U];
-        int part[U];
+    // the wavefront's first record, the one after it, and the first
+    // record's first dense A fragments, in flight with the window (its panel
+    // K-step is issued after the barrier, ahead of the first dense MFMAs)
+    uint32_t c0, c1, n0, n1;
+    load_rec(nrun > 0 ? __builtin_amdgcn_readlane(sv, 0) : NONE, c0, c1);
+    int nsi = 0;
+    uint32_t noff = nrun > 0 ? next_of(c0, c1, 0, nsi) : NONE;
+    load_rec(noff, n0, n1);
+    AOp A, B, C;
+    POp PN;
+    int nd = nrun > 0 ? static_cast<int>(rword(c0, c1, 2)) : 0;
+    issue_a(c0, c1, 0, nd > 0, A);
+    issue_a(c0, c1, 1, nd > 1, B);
+    if constexpr (STAGED) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int p = threadIdx.x + 64 * SW * u;
+            const int p = threadIdx.x + 64 * NW * u;
             const int c = p / NQ, q = p % NQ;
-            const int64_t row = static_cast<int64_t>(cmin) + 4 * q;
-            const bool in = c < ncol && 4 * q < W;
-            part[u] = in ? static_cast<int>(min<int64_t>(4, ncols - row)) : 0;
-            const float *src = part[u] == 4 ? X + static_cast<int64_t>(c) * ldx + row : zeros;
-            v[u] = *reinterpret_cast<const f4u *>(src);
-        }
-        decode();
-        uint4 q0, q1;
-        bf16x8 pah, pal, ah[KC], al[KC];
-        first_loads(q0, q1, pah, pal, ah, al);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (part[u] > 0 && part[u] < 4) {        // the quad cut by the last row of X
-                const int p = threadIdx.x + 64 * SW * u;
-                const float *src = X + static_cast<int64_t>(p / NQ) * ldx + cmin + 4 * (p % NQ);
+            const int part = unit_rows(u);
+            if (part > 0 && part < 4) {              // the quad cut by the last row of X
+                const float *src = X + static_cast<int64_t>(c) * ldx + cmin + 4 * q;
                 f4u w = {0.f, 0.f, 0.f, 0.f};
-                for (int e = 0; e < part[u]; ++e) w[e] = src[e];
-                v[u] = w;
+                for (int e = 0; e < part; ++e) w[e] = src[e];
+                xv[u] = w;
+            }
+            uint32_t hi0, lo0, hi1, lo1;
+            split2(xv[u][0], xv[u][1], hi0, lo0);
+            split2(xv[u][2], xv[u][3], hi1, lo1);
+            if (p < NB * NQ) {
+                *(reinterpret_cast<uint2 *>(&xs[0][c][q >> 1]) + (q & 1)) = uint2{hi0, hi1};
+                *(reinterpret_cast<uint2 *>(&xs[1][c][q >> 1]) + (q & 1)) = uint2{lo0, lo1};
             }
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int p = threadIdx.x + 64 * SW * u;
-            *reinterpret_cast<float4 *>(&xs[(p / NQ) * XS_LD + 4 * (p % NQ)]) =
-                make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
-        }
-        if (threadIdx.x < 8) xs[NB * XS_LD + threadIdx.x] = 0.0f;
-        __syncthreads();
-        if (!has) return;
-        first_compute(std::true_type{}, q0, q1, pah, pal, ah, al);
-        finish(std::true_type{});
-    } else {
-        if (!has) return;
-        decode();
-        uint4 q0, q1;
-        bf16x8 pah, pal, ah[KC], al[KC];
-        first_loads(q0, q1, pah, pal, ah, al);
-        first_compute(std::false_type{}, q0, q1, pah, pal, ah, al);
-        finish(std::false_type{});
     }
+    __syncthreads();
+    if (nrun == 0) return;
+    issue_panel(c0, c1, nd, rword(c0, c1, 3) > 0u, PN);
+
+    auto walk = [&](auto staged) __attribute__((always_inline)) {
+        for (;;) {
+            // dense K-steps in groups of 3 through the sets A, B, C: the A
+            // fragments of K-step k + 2 are loaded before the MFMAs of k
+            const int trips = (max(nd, 1) + 2) / 3;
+            for (int q = 0; q + 1 < trips; ++q) {
+                const int k = 3 * q;
+                issue_a(c0, c1, k + 2, k + 2 < nd, C);
+                __builtin_amdgcn_sched_barrier(0);      // C's loads stay ahead of A's MFMAs
+                compute_dense(c0, c1, k, true, A, staged);
+                issue_a(c0, c1, k + 3, k + 3 < nd, A);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_dense(c0, c1, k + 1, k + 1 < nd, B, staged);
+                issue_a(c0, c1, k + 4, k + 4 < nd, B);
+                __builtin_amdgcn_sched_barrier(0);
+                compute_dense(c0, c1, k + 2, k + 2 < nd, C, staged);
+            }
+            // last group: its third step, then the next record's loads
+            const int kl = 3 * (trips - 1);
+            issue_a(c0, c1, kl + 2, kl + 2 < nd, C);
+            __builtin_amdgcn_sched_barrier(0);
+            compute_dense(c0, c1, kl, kl < nd, A, staged);
+            compute_dense(c0, c1, kl + 1, kl + 1 < nd, B, staged);
+            compute_panel(PN);
+            const bool more = noff != NONE;
+            int nnsi = nsi;
+            const uint32_t nnoff = more ? next_of(n0, n1, nsi, nnsi) : NONE;
+            uint32_t m0, m1;
+            load_rec(nnoff, m0, m1);
+            const int nnd = more ? static_cast<int>(rword(n0, n1, 2)) : 0;
+            {
+                const int nnp = more ? static_cast<int>(rword(n0, n1, 3)) : 0;
+                issue_panel(n0, n1, nnd, more && nnp > 0, PN);
+                issue_a(n0, n1, 0, more && nnd > 0, A);
+                issue_a(n0, n1, 1, more && nnd > 1, B);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            compute_dense(c0, c1, kl + 2, kl + 2 < nd, C, staged);
+            if (rword(c0, c1, 5) == 0u) store(static_cast<int>(rword(c0, c1, 0)), static_cast<int>(rword(c0, c1, 1)));
+            if (!more) break;
+            c0 = n0;
+            c1 = n1;
+            n0 = m0;
+            n1 = m1;
+            nsi = nnsi;
+            noff = nnoff;
+            nd = nnd;
+        }
+    };
+    walk(std::integral_constant<bool, STAGED>{});
 }
 
-
 template <int MODE>
-static hipError_t run_b(const Direction &d, const void *zeros, int ncol, const float *X,
-                        int64_t ldx, float *Y, int64_t ldy, float alpha, float beta, hipStream_t s,
-                        hipEvent_t *ev) {
+static hipError_t run_b(const Plan &p, const Direction &d, int ncol, const float *X, int64_t ldx, float *Y,
+                        int64_t ldy, float alpha, float beta, hipStream_t s, hipEvent_t *ev) {
     const Batched &b = d.bat;
+    const float *zf = static_cast<const float *>(p.zeros);
     hipError_t e = hipSuccess;
     if (b.nst > 0 && b.gps > 0) {
-        hipLaunchKernelGGL(kb_proj<MODE>, dim3(static_cast<unsigned>(b.nst)), dim3(256), 0, s,
-                           b.gps, b.pcol, b.vslc, static_cast<const float *>(zeros), X, ldx,
-                           d.ncols, ncol, b.t);
+        hipLaunchKernelGGL(kb_proj<MODE>, dim3(static_cast<unsigned>(b.nst)), dim3(256), 0, s, b.gps, b.pcol,
+                           b.vslc, zf, X, ldx, d.ncols, ncol, b.t);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -664,29 +690,37 @@ static hipError_t run_b(const Direction &d, const void *zeros, int ncol, const f
     if (b.nst > 0 && b.gps > 0 && b.nred > 0) {
         const int64_t nwg = b.nredbig + (b.nred - b.nredbig + 3) / 4;
         hipLaunchKernelGGL(kb_reduce, dim3(static_cast<unsigned>(nwg)), dim3(256), 0, s, b.red, b.nredbig, b.nred,
-                           static_cast<const float *>(b.t) + b.tLen, static_cast<const float *>(zeros), b.t);
+                           static_cast<const float *>(b.t) + b.tLen, zf, b.t);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (ev) (void)hipEventRecord(ev[1], s);
     if (b.nstrips > 0) {
-        const int sw = b.shape.sw;
-        const int64_t nwg = (b.nstrips + sw - 1) / sw;
+        const int64_t band = static_cast<int64_t>(b.shape.nw) * b.shape.spw;
+        const int64_t nwg = (b.nstrips + band - 1) / band;
         const int64_t per = (nwg + 7) / 8;        // workgroups per XCD
-        const dim3 grid(static_cast<unsigned>(8 * per)), block(static_cast<unsigned>(64 * sw));
-        const float *zf = static_cast<const float *>(zeros);
+        const dim3 grid(static_cast<unsigned>(8 * per)), block(static_cast<unsigned>(64 * b.shape.nw));
         const float *tf = static_cast<const float *>(b.t);
         const int bz = beta == 0.0f ? 1 : 0;
-        if (sw == ROW_WIDE.sw && b.shape.xsm == ROW_WIDE.xsm)
-            hipLaunchKernelGGL((kb_strips<MODE, SKC, ROW_WIDE.sw, ROW_WIDE.xsm>), grid, block, 0, s, b.nstrips,
-                               per, b.soff, b.recs, b.wgx, b.frag, zf, tf, X, ldx, d.ncols, Y, ldy, ncol, alpha,
-                               beta, bz);
-        else if (sw == ROW_NARROW.sw && b.shape.xsm == ROW_NARROW.xsm)
-            hipLaunchKernelGGL((kb_strips<MODE, SKC, ROW_NARROW.sw, ROW_NARROW.xsm>), grid, block, 0, s,
-                               b.nstrips, per, b.soff, b.recs, b.wgx, b.frag, zf, tf, X, ldx, d.ncols, Y, ldy, ncol,
-                               alpha, beta, bz);
-        else
-            return hipErrorInvalidValue;
+        // bands with a staged window, then (if the plan has any) the others
+        for (int staged = 1; staged >= 0; --staged) {
+            if (!staged && b.nunstaged == 0) break;
+            const bool wide = b.shape.nw == ROW_WIDE.nw && b.shape.spw == ROW_WIDE.spw && b.shape.xsm == ROW_WIDE.xsm;
+            const bool narrow =
+                b.shape.nw == ROW_NARROW.nw && b.shape.spw == ROW_NARROW.spw && b.shape.xsm == ROW_NARROW.xsm;
+            if (!wide && !narrow) return hipErrorInvalidValue;
+            const dim3 g2 = staged ? grid : dim3(static_cast<unsigned>(b.nunstaged));
+#define HM_ROWS(SH, ST)                                                                                          \
+    hipLaunchKernelGGL((kb_rows<MODE, SH.nw, SH.spw, SH.xsm, ST>), g2, block, 0, s, b.nstrips, per, b.soff, b.recs, \
+                       b.wgx, b.ulist, b.frag, zf, tf, X, ldx, d.ncols, Y, ldy, ncol, alpha, beta, bz)
+            if (wide && staged) HM_ROWS(ROW_WIDE, true);
+            else if (wide) HM_ROWS(ROW_WIDE, false);
+            else if (staged) HM_ROWS(ROW_NARROW, true);
+            else HM_ROWS(ROW_NARROW, false);
+#undef HM_ROWS
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -698,9 +732,9 @@ hipError_t launch_batched(const Plan &p, const Direction &d, int ncol, const flo
                           float *Y, int64_t ldy, float alpha, float beta, hipStream_t s,
                           hipEvent_t *ev) {
     switch (p.storage) {
-        case MODE_F32: return run_b<MODE_F32>(d, p.zeros, ncol, X, ldx, Y, ldy, alpha, beta, s, ev);
-        case MODE_BF16: return run_b<MODE_BF16>(d, p.zeros, ncol, X, ldx, Y, ldy, alpha, beta, s, ev);
-        case MODE_BF16X2: return run_b<MODE_BF16X2>(d, p.zeros, ncol, X, ldx, Y, ldy, alpha, beta, s, ev);
+        case MODE_F32: return run_b<MODE_F32>(p, d, ncol, X, ldx, Y, ldy, alpha, beta, s, ev);
+        case MODE_BF16: return run_b<MODE_BF16>(p, d, ncol, X, ldx, Y, ldy, alpha, beta, s, ev);
+        case MODE_BF16X2: return run_b<MODE_BF16X2>(p, d, ncol, X, ldx, Y, ldy, alpha, beta, s, ev);
         default: return hipErrorInvalidValue;
     }
 }

@@ -89,20 +89,20 @@ constexpr int PROJ_SUPER = 256;
 // straight from global memory: X rows [src, src + 32) for a dense piece, or
 // 32 gathered rows of t for the strip's panel of low-rank columns.
 constexpr int SROWS = 16;
-// Row-pass workgroup shapes: sw strips (one wavefront each) share a window
-// of at most xsm rows of X (all NB columns) staged in LDS with row stride
-// xsm + 4 (stride / 4 odd keeps the 16-byte reads of 8 consecutive lanes, 8
-// columns of X, on distinct bank groups); a workgroup whose dense pieces
-// span more rows reads X straight from memory.  The plan takes the narrow
-// shape (two workgroups per CU) unless more than a tenth of its workgroups
-// would not stage their window; then the wide one (one workgroup of 256
-// rows per CU), for leaves that reach far from the diagonal (strong
-// admissibility).
+// Row-pass workgroup shapes: nw wavefronts (one per strip at a time) take
+// spw strips each of a band of nw * spw strips, sharing the band's X window
+// of at most xsm rows (all NB columns), staged in LDS as split BF16; a band
+// whose dense pieces span more rows reads X straight from memory.  The plan
+// takes the narrow shape (two workgroups of 256 rows per CU) unless more
+// than a tenth of its bands would not stage their window; then the wide one
+// (one workgroup of 512 rows per CU), for leaves that reach far from the
+// diagonal (strong admissibility).  xsm / 8 is even: the LDS octet stride
+// xsm / 8 + 1 is odd.
 struct RowShape {
-    int sw, xsm;
+    int nw, spw, xsm;
 };
-constexpr RowShape ROW_NARROW = {8, 512};
-constexpr RowShape ROW_WIDE = {16, 1024};
+constexpr RowShape ROW_NARROW = {8, 2, 528};
+constexpr RowShape ROW_WIDE = {16, 2, 1056};
 
 // Strip record, 32-bit words, 16-byte aligned:
 //   0 row0, 1 rows (<= 16), 2 nd = dense K-steps, 3 np = panel K-steps,
@@ -153,10 +153,12 @@ struct BRed {
 struct Batched {
     bool ready = false;
     int64_t nstrips = 0;
-    RowShape shape = ROW_NARROW;  // row-pass workgroup shape
     uint32_t *soff = nullptr;     // word offset of each strip record
     uint32_t *recs = nullptr;     // strip records
-    int32_t *wgx = nullptr;       // per workgroup: X window (first row, rows; 0 rows: not staged)
+    RowShape shape = ROW_NARROW;  // row-pass workgroup shape
+    int32_t *wgx = nullptr;       // per band: X window (first row, rows; 0 rows: not staged)
+    int64_t nunstaged = 0;        // bands whose window is not staged
+    uint32_t *ulist = nullptr;    // their indices
     uint4 *pcol = nullptr;        // projection column descriptors [nst][gps][16]
     int64_t nst = 0, gps = 0;     // super-tiles, groups per super-tile
     BRed *red = nullptr;          // largest first

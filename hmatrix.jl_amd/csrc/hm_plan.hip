@@ -311,21 +311,22 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
         }
         std::vector<Piece>().swap(ps);
     }
-    // Workgroup X windows: the sw strips of a workgroup read their dense
-    // pieces' X rows from one window [cmin, cmin + W) staged in LDS when the
-    // window (pieces started at a multiple of 4, rounded up to octets) is at
-    // most xsm rows wide; otherwise (W = 0) straight from global memory.
-    // Returns the number of workgroups with dense pieces and no window.
+    // Band X windows: the strips of a band read their dense pieces' X rows
+    // from one window [cmin, cmin + W) staged in LDS when the window (pieces
+    // from a multiple of 8 to their last whole octet) is at most xsm rows
+    // wide; otherwise (W = 0) straight from global memory.  Returns the
+    // number of bands with dense pieces and no window.
     std::vector<int32_t> wgx;
     auto windows = [&](const RowShape &sh) {
-        const int64_t nwg = (nstrips + sh.sw - 1) / sh.sw;
+        const int64_t band = static_cast<int64_t>(sh.nw) * sh.spw;
+        const int64_t nwg = (nstrips + band - 1) / band;
         wgx.assign(static_cast<size_t>(2 * nwg), 0);
         int64_t wide = 0;
         for (int64_t w = 0; w < nwg; ++w) {
             int64_t cmin = INT64_MAX, cmax = -1;
-            for (int64_t t = w * sh.sw; t < std::min(nstrips, (w + 1) * sh.sw); ++t)
+            for (int64_t t = w * band; t < std::min(nstrips, (w + 1) * band); ++t)
                 for (const Piece &pc : merged[t]) {
-                    const int64_t a0 = pc.c0 & ~int64_t(3);
+                    const int64_t a0 = pc.c0 & ~int64_t(7);
                     cmin = std::min(cmin, a0);
                     cmax = std::max(cmax, a0 + ((pc.c0 + pc.n - a0 + 7) & ~int64_t(7)));
                 }
@@ -347,24 +348,22 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
     for (int64_t t = 0; t < nstrips; ++t) {
         const int64_t srow0 = t * SROWS;
         const int32_t R = static_cast<int32_t>(std::min<int64_t>(SROWS, nrows - srow0));
-        const bool staged = wgx[2 * (t / shape.sw) + 1] > 0;
         const std::vector<Piece> &mg = merged[t];
-        std::vector<uint32_t> dsc, pan, idxw;     // dense / panel descriptors, panel index blocks
-        int32_t nd = 0, np = 0;
-        auto add_step = [&](std::vector<uint32_t> &to, uint32_t afrag, uint32_t src, int roff, int rows,
-                            int nOct, int kind) {
-            to.push_back(afrag);
-            to.push_back(src);
-            to.push_back(static_cast<uint32_t>(roff | (rows << 4) | (nOct << 9) | (kind << 12)));
-            to.push_back(0u);
+        // K-steps: {afrag, src, info}; dense ones first, then the panel ones
+        struct KStep {
+            uint32_t afrag, src, info;
         };
+        std::vector<KStep> ks;
+        std::vector<uint32_t> idxw;     // 32 t-row indices per panel K-step
+        int32_t nd = 0;
         // K-steps of X columns [x0, x0 + w) of one merged piece whose piece
-        // column kk sits at X column pc.c0 + kk; only piece columns
-        // [klo, pc.n) are taken, every other column holds zero A values.
-        auto emit = [&](const Piece &pc, int64_t x0, int64_t w, int64_t klo) {
+        // column kk sits at X column pc.c0 + kk.  x0 is a multiple of 8: the
+        // row pass reads whole octets of the split X tile, which is zero past
+        // the last row of X.
+        auto emit = [&](const Piece &pc, int64_t x0, int64_t w) {
             auto val = [&](int r, int64_t j) {
                 const int64_t kk = x0 + j - pc.c0;
-                if (kk < klo || kk >= pc.n) return 0.0;
+                if (kk < 0 || kk >= pc.n) return 0.0;
                 size_t q = pc.parts.size() - 1;
                 while (pc.parts[q].second > kk) --q;
                 const OB &o = obs[pc.parts[q].first];
@@ -376,34 +375,14 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
                 const int64_t rem = w - 32 * s;
                 const int nOct = rem >= 32 ? 4 : static_cast<int>((rem + 7) / 8);
                 const uint32_t af = pack(pc.rows, s, w, nOct, val);
-                add_step(dsc, af, static_cast<uint32_t>(x0 + 32 * s), pc.roff, pc.rows, nOct, 0);
+                ks.push_back(KStep{af, static_cast<uint32_t>(x0 + 32 * s),
+                                   static_cast<uint32_t>(pc.roff | (pc.rows << 4) | (nOct << 9))});
                 ++nd;
             }
         };
         for (const Piece &pc : mg) {
-            if (staged) {
-                // from the LDS window: start on a multiple of 4 (16-byte LDS
-                // reads); rows past ncols are zeros there
-                const int64_t a0 = pc.c0 & ~int64_t(3);
-                emit(pc, a0, pc.c0 + pc.n - a0, 0);
-                continue;
-            }
-            // Every octet of X rows a K-step reads must end by ncols: a piece
-            // whose rounded-up end passes ncols starts up to 7 columns early
-            // (zero A columns); if it cannot (it starts within those 7
-            // columns), its whole octets are one run and its last columns a
-            // second run over the final 8 columns of X.
-            const int64_t n8 = (pc.n + 7) & ~int64_t(7);
-            const int64_t over = pc.c0 + n8 - ncols;
-            if (over <= 0) {
-                emit(pc, pc.c0, pc.n, 0);
-            } else if (pc.c0 >= over) {
-                emit(pc, pc.c0 - over, pc.n + over, 0);
-            } else {
-                const int64_t whole = pc.n & ~int64_t(7);
-                if (whole > 0) emit(pc, pc.c0, whole, 0);
-                emit(pc, ncols - 8, 8, whole);
-            }
+            const int64_t a0 = pc.c0 & ~int64_t(7);
+            emit(pc, a0, pc.c0 + pc.n - a0);
         }
         // the panel of low-rank columns: A rows of every leaf crossing the strip
         std::vector<std::pair<int32_t, int64_t>> ucols;     // (leaf, rank column)
@@ -418,34 +397,44 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
                 const int64_t lr = srow0 + r - o.row0;
                 return (lr >= 0 && lr < o.m) ? fac[o.offA + ucols[kk].second * o.m + lr] : 0.0;
             });
-            add_step(pan, af, 0u, 0, R, nOct, SK_KIND_PANEL);
-            ++np;
+            ks.push_back(KStep{af, 0u, static_cast<uint32_t>((R << 4) | (nOct << 9) | (SK_KIND_PANEL << 12))});
             for (int e = 0; e < 32; ++e) {
                 const int64_t kk = 32 * s + e;
                 const int64_t row = kk < nu ? (tOff[ucols[kk].first] + ucols[kk].second * NB) / NB : zrow;
                 idxw.push_back(static_cast<uint32_t>(row));
             }
         }
-        // Records of at most SREC_MAXK K-steps (dense first, then panel), each
-        // followed by the index blocks of its panel K-steps (a block of
-        // zero-row indices when it has none), chained through word 5.
-        const int32_t ntot = nd + np;
+        // Records: header, 4 words per K-step (dense ones first), then the
+        // 32-word index block of the record's panel K-step (at most one), all
+        // within the first 128 words (the row pass reads index words from the
+        // two record registers); a strip with more K-steps continues in a
+        // chained record (word 5).
+        const int32_t ntot = static_cast<int32_t>(ks.size());
         soff[t] = static_cast<uint32_t>(recs.size());
-        for (int32_t k0 = 0; k0 == 0 || k0 < ntot; k0 += SREC_MAXK) {
-            const int32_t k1 = std::min(ntot, k0 + SREC_MAXK);
-            const int32_t rd = std::max(0, std::min(nd, k1) - k0);     // dense steps in this record
-            const int32_t rpn = (k1 - k0) - rd;                          // panel steps
-            const int32_t p0 = std::max(0, k0 - nd);                     // first panel step
+        int32_t k0 = 0;
+        do {
+            int32_t k1 = k0, rd = 0, rpn = 0;
+            while (k1 < ntot) {
+                const bool panel = k1 >= nd;
+                const int32_t words = SREC_HDR + 4 * (k1 + 1 - k0) + 32 * (rpn + (panel ? 1 : 0));
+                if (words > 128 || (panel && rpn == 1)) break;     // one panel K-step per record
+                if (panel) ++rpn;
+                else ++rd;
+                ++k1;
+            }
             std::vector<uint32_t> rec(SREC_HDR, 0u);
-            rec.insert(rec.end(), dsc.begin() + 4 * std::min(k0, nd), dsc.begin() + 4 * std::min(k1, nd));
-            const size_t pd = rec.size();
-            rec.insert(rec.end(), pan.begin() + 4 * p0, pan.begin() + 4 * (p0 + rpn));
+            for (int32_t k = k0; k < k1; ++k) {
+                rec.push_back(ks[k].afrag);
+                rec.push_back(ks[k].src);
+                rec.push_back(ks[k].info);
+                rec.push_back(0u);
+            }
             const size_t ib = rec.size();
             for (int32_t q = 0; q < rpn; ++q) {
-                rec[pd + 4 * q + 1] = static_cast<uint32_t>(ib + 32 * q);
-                rec.insert(rec.end(), idxw.begin() + 32 * (p0 + q), idxw.begin() + 32 * (p0 + q + 1));
+                const int32_t k = k0 + rd + q;            // panel K-step k is panel number k - nd
+                rec[SREC_HDR + 4 * (rd + q) + 1] = static_cast<uint32_t>(ib + 32 * q);
+                rec.insert(rec.end(), idxw.begin() + 32 * (k - nd), idxw.begin() + 32 * (k - nd + 1));
             }
-            if (rpn == 0) rec.insert(rec.end(), 32, static_cast<uint32_t>(zrow));
             rec[0] = static_cast<uint32_t>(srow0);
             rec[1] = static_cast<uint32_t>(R);
             rec[2] = static_cast<uint32_t>(rd);
@@ -453,10 +442,11 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
             rec[4] = static_cast<uint32_t>(ib);
             while (rec.size() % 4) rec.push_back(0u);
             const size_t at = recs.size();
-            if (at + rec.size() >= (size_t(1) << 32)) throw std::length_error("strip records exceed 2^32 words");
+            if (at + rec.size() + 128 >= (size_t(1) << 32)) throw std::length_error("strip records exceed 2^32 words");
             if (k1 < ntot) rec[5] = static_cast<uint32_t>(at + rec.size());   // next record
             recs.insert(recs.end(), rec.begin(), rec.end());
-        }
+            k0 = k1;
+        } while (k0 < ntot);
     }
     // ---- projection: super-tiles of PROJ_SUPER columns.  The rank columns
     // (leaf, j) crossing a super-tile are cut into groups of 16; each column
@@ -527,7 +517,7 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
     H.soff = std::move(soff);
     H.recs = std::move(recs);
     H.pcol = std::move(pcol);
-    H.nst = gps > 0 ? nst : 0;
+    H.nst = nst;
     H.gps = gps;
     H.red = std::move(red);
     H.fr = std::move(fr);
@@ -544,10 +534,17 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
     if (!layout_batched(p.storage, p.strip_shape, nrows, ncols, obs, fac, dense, H)) return;
     Batched &B = d.bat;
     B.nstrips = H.nstrips;
-    B.shape = H.shape;
     B.soff = dev_upload(p, H.soff);
     B.recs = dev_upload(p, H.recs);
+    B.shape = H.shape;
     B.wgx = dev_upload(p, H.wgx);
+    {
+        std::vector<uint32_t> ul;
+        for (size_t w = 0; 2 * w + 1 < H.wgx.size(); ++w)
+            if (H.wgx[2 * w + 1] == 0) ul.push_back(static_cast<uint32_t>(w));
+        B.nunstaged = static_cast<int64_t>(ul.size());
+        B.ulist = dev_upload(p, ul);
+    }
     B.pcol = H.pcol.empty() ? nullptr : reinterpret_cast<uint4 *>(dev_upload(p, H.pcol));
     B.nst = H.nst;
     B.gps = H.gps;
@@ -1054,8 +1051,8 @@ int hm_plan_destroy(hm_plan_t plan) {
 // enum hm_strip_shape of the forward batched row pass (0: no batched layout)
 int64_t strip_shape_in_use(const hm::Plan &p) {
     const hm::Batched &b = p.fwd.bat;
-    if (b.nstrips == 0) return HM_STRIPS_AUTO;
-    return b.shape.sw == hm::ROW_WIDE.sw ? HM_STRIPS_WIDE : HM_STRIPS_NARROW;
+    if (!b.ready) return HM_STRIPS_AUTO;
+    return b.shape.nw == hm::ROW_WIDE.nw ? HM_STRIPS_WIDE : HM_STRIPS_NARROW;
 }
 
 int hm_plan_info(hm_plan_t plan, int64_t *out, int nout) {
