@@ -2,7 +2,8 @@
 
 Host side (FP64, numpy): ``Hparams``, ``NewHmat``, ``FullMat``, ``Hmat``,
 ``Cluster``, ``aca2``, ``info``, ``Array``/``to_fmat`` -- the names of
-src/cluster.jl:1-5, src/utils.jl:2-6 and src/hlu.jl:7.
+src/cluster.jl:1-5, src/utils.jl:2-6 and src/hlu.jl:7 -- and the H-LU
+``lu`` / ``lu_inplace`` (``lu!``) / ``hsolve`` (``\``) of src/hlu.jl:465-561.
 Device side (HIP, gfx950): ``H * y``, ``H * X``, ``v * H``, ``H.T * y`` and
 ``DeviceHmat`` for device-resident products in FP64, FP32, BF16 or split
 BF16 storage.
@@ -13,7 +14,8 @@ The directory is named ``hmatrix.jl_amd``; it is imported as
 from .params import Hparams, Params
 from .cluster import (Cluster, Hmat, NewHmat, FullMat, kernel_full, construct_cluster, bisect_cluster,
                       UniformScaling, I, copy_hmat)
-from .solve import gmres, bicgstab, solve_ir
+from .solve import gmres, bicgstab, solve_ir, factorized
+from .hlu import lu, lu_inplace, solve as hsolve, hmul, hadd, getl, getu
 from .lowrank import aca2, compress, rank_truncate, bbfmm
 from .flat import (FlatHmat, flatten, to_dense, to_fmat, Array, info, save_factors, load_factors,
                    KIND_DENSE, KIND_RK)
@@ -23,7 +25,8 @@ __all__ = [
     "construct_cluster", "bisect_cluster", "aca2", "compress", "rank_truncate", "bbfmm",
     "FlatHmat", "flatten", "to_dense", "to_fmat", "Array", "info", "save_factors",
     "load_factors", "KIND_DENSE", "KIND_RK", "DeviceHmat", "hmatvec", "UniformScaling", "I",
-    "copy_hmat", "gmres", "bicgstab", "solve_ir",
+    "copy_hmat", "gmres", "bicgstab", "solve_ir", "factorized", "lu", "lu_inplace", "hsolve", "hmul", "hadd",
+    "getl", "getu",
 ]
 
 

@@ -6,11 +6,13 @@
   same warnings on breakdown / non-convergence.
 * ``solve_ir``: ``H\\b`` by mixed-precision iterative refinement.  The
   residual ``r = b - H x`` is accumulated in FP64 on the GPU (the plain FP64
-  product); corrections come from restarted GMRES on the device (basis
-  vectors in HBM) over a low-precision copy of H (FP32 or split-BF16
-  storage).  The reference's ``\\`` (src/hlu.jl:553-561) needs an
-  H-LU factorisation (``lu!``, src/hlu.jl:465-515), which is not part of this
-  library; refinement reaches the same solution to the requested tolerance.
+  product); corrections come either from the host H-LU factorisation in FP64
+  (``inner="hlu"``: the reference's ``lu!`` and ``\\``, src/hlu.jl:465-561,
+  module ``hlu``) or from restarted GMRES on the device (basis vectors in
+  HBM) over a low-precision copy of H (FP32 or split-BF16 storage).
+* ``M`` (preconditioner) of ``gmres`` / ``bicgstab`` may be an ``Hmat``: its
+  host H-LU factors (computed once if it is not factorised yet) apply
+  ``M\\x``.
 """
 from __future__ import annotations
 
@@ -19,6 +21,7 @@ from typing import Callable, Optional
 
 import numpy as np
 
+from . import hlu
 from .cluster import Hmat
 
 
@@ -45,8 +48,22 @@ def _precond(M, N: int):
     if isinstance(M, np.ndarray):
         return ssl.LinearOperator((N, N), matvec=lambda x: np.linalg.solve(M, x), dtype=np.float64)
     if isinstance(M, Hmat):
-        raise NotImplementedError("an Hmat preconditioner needs M\\x (H-LU), which this library does not provide")
+        LU = factorized(M)
+        return ssl.LinearOperator((N, N), matvec=lambda x: hlu.solve(LU, np.asarray(x, dtype=np.float64).ravel()),
+                                  dtype=np.float64)
     return ssl.LinearOperator((N, N), matvec=M, dtype=np.float64)
+
+
+def factorized(H: Hmat) -> Hmat:
+    """H itself when it holds its H-LU factors (``lu!`` was applied), else
+    the factors of a copy, computed once and kept with H."""
+    if len(H.P) > 0:
+        return H
+    LU = getattr(H, "_hlu", None)
+    if LU is None:
+        LU = hlu.lu(H)
+        H._hlu = LU
+    return LU
 
 
 def _report(info: int) -> None:
@@ -140,18 +157,30 @@ def _gmres_device(mv, b, tol: float, restart: int, maxiter: int):
 
 def solve_ir(H, b, tol: float = 1e-12, inner: str = "f32", inner_tol: float = 1e-4, maxiter: int = 30,
              restart: int = 50):
-    """Solve ``H x = b`` by iterative refinement, entirely on the device.
+    """Solve ``H x = b`` by iterative refinement.
 
-    Each sweep: ``r = b - H x`` with the FP64 GPU product, then ``H d = r``
-    by restarted GMRES (relative tolerance ``inner_tol``, at most 20
-    restarts) on the ``inner`` storage copy of H, with the Krylov basis on
-    the device, then ``x += d``.  Stops when ``||r|| <= tol * ||b||``.
-    Returns ``(x, residual_history)``.
+    Each sweep: ``r = b - H x`` with the FP64 GPU product, then a correction
+    ``d ~ H \\ r`` and ``x += d``.  ``inner="hlu"``: d from the host FP64
+    H-LU factors of H (``lu!`` and ``\\``, src/hlu.jl:465-561; computed
+    once and kept with H, which must be an ``Hmat``).  Otherwise ``inner`` is
+    a device storage ("f32", "bf16x2", ...) and d comes from restarted GMRES
+    (relative tolerance ``inner_tol``, at most 20 restarts) on that copy of
+    H, with the Krylov basis on the device.  Stops when
+    ``||r|| <= tol * ||b||``.  Returns ``(x, residual_history)``.
     """
     import torch
     from .device import DeviceHmat
+    if isinstance(H, Hmat) and len(H.P) > 0:
+        raise ValueError("H holds its own H-LU factors (lu! was applied): solve_ir needs the matrix for "
+                         "the residual; keep an unfactorised copy, or use hsolve")
     dev64 = H.device("f64") if isinstance(H, Hmat) else H
-    if isinstance(H, Hmat):
+    LU = None
+    if inner == "hlu":
+        if not isinstance(H, Hmat):
+            raise TypeError("inner='hlu' needs the host Hmat (its H-LU factors are computed on the host)")
+        LU = factorized(H)
+        devlo = None
+    elif isinstance(H, Hmat):
         devlo = H.device(inner)
     else:
         devlo = DeviceHmat(dev64.flat, inner)
@@ -162,14 +191,16 @@ def solve_ir(H, b, tol: float = 1e-12, inner: str = "f32", inner_tol: float = 1e
     rd = torch.empty_like(bd)
     nb = float(np.linalg.norm(b))
     hist = []
-    mv = devlo.matvec
     for _ in range(maxiter + 1):
         rd.copy_(bd)
         dev64.matvec(xd, rd, alpha=-1.0, beta=1.0)          # r = b - H x, FP64
         hist.append(float(torch.linalg.norm(rd)) / nb if nb > 0 else 0.0)
         if hist[-1] <= tol:
             break
-        d, _ok = _gmres_device(mv, rd.to(devlo.vec_dtype), inner_tol, restart, 20)
+        if LU is not None:
+            d = torch.from_numpy(hlu.solve(LU, rd.cpu().numpy())).to(xd.device)
+        else:
+            d, _ok = _gmres_device(devlo.matvec, rd.to(devlo.vec_dtype), inner_tol, restart, 20)
         xd += d.to(torch.float64)
     else:
         warnings.warn("iterative refinement did not reach the tolerance")
