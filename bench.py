@@ -228,7 +228,9 @@ def measure_batched(hm, torch, f, RHS, storage, args, streams, rows, Yd):
 def c4_report(hm, torch, stream) -> dict:
     """64 H-matrices, N log-uniform in [1000, 40000], 1/(1+((x-y)/l)^2),
     l ~ U[0.2, 5], MaxRank in {4, 8, 16}; FP32; one launch (captured in a
-    HIP graph) for all members."""
+    HIP graph) for all members: three launches per product (projection,
+    reduction, row tiles), and beside it the one-launch form
+    (HM_ONE_LAUNCH)."""
     from hmatrix_amd.workloads import uneven_batch
     t0 = time.time()
     members = uneven_batch(hm, nmat=64, nlo=1000, nhi=40000, seed=6)
@@ -239,17 +241,28 @@ def c4_report(hm, torch, stream) -> dict:
     g = d.graph(x, y)
     ms = timed(torch, lambda i: g.launch(stream), 50, 5, stream)
     ms_eager = timed(torch, lambda i: d.matvec(x, y, stream=stream), 50, 5, stream)
-    elems = sum(m.stored_elements() for m in members)
-    nbytes = 4 * elems + 8 * d.nrows
-    log(f"C4 batch (64 matrices, FP32, 1 rhs each): graph {ms * 1e3:.1f} us, eager {ms_eager * 1e3:.1f} us "
-        f"per launch; {64 / (ms * 1e-3):.0f} H-matvec/s; {nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
     g.close()
     d.close()
+    # the same batch as one launch per product
+    d1 = hm.DeviceHmat.batch(members, "f32", {"no_transpose": True, "no_batched": True, "one_launch": True})
+    assert d1.info()["fused"] == 1
+    g1 = d1.graph(x, y)
+    ms1 = timed(torch, lambda i: g1.launch(stream), 50, 5, stream)
+    ms1_eager = timed(torch, lambda i: d1.matvec(x, y, stream=stream), 50, 5, stream)
+    assert d1.info()["wait_faults"] == 0
+    g1.close()
+    d1.close()
+    elems = sum(m.stored_elements() for m in members)
+    nbytes = 4 * elems + 8 * d.nrows
+    log(f"C4 batch (64 matrices, FP32, 1 rhs each), three launches: graph {ms * 1e3:.1f} us, eager "
+        f"{ms_eager * 1e3:.1f} us; one launch: graph {ms1 * 1e3:.1f} us, eager {ms1_eager * 1e3:.1f} us; "
+        f"{64 / (ms_eager * 1e-3):.0f} H-matvec/s; {nbytes / (ms_eager * 1e-3) / 1e9:.0f} GB/s (eager)")
     return {"workload": "64 H-matrices, N log-uniform in [1000, 40000] (seed 6), 1/(1+((x-y)/l)^2), "
-                        "MaxRank in {4,8,16}, FP32, one right-hand side each, one launch",
+                        "MaxRank in {4,8,16}, FP32, one right-hand side each, all members per launch",
             "total_N": int(d.nrows), "graph_us": ms * 1e3, "eager_us": ms_eager * 1e3,
-            "H-matvec/s": 64 / (ms * 1e-3), "algorithmic_bytes": nbytes,
-            "GB/s": nbytes / (ms * 1e-3) / 1e9, "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "one_launch_graph_us": ms1 * 1e3, "one_launch_eager_us": ms1_eager * 1e3,
+            "H-matvec/s": 64 / (ms_eager * 1e-3), "algorithmic_bytes": nbytes,
+            "GB/s": nbytes / (ms_eager * 1e-3) / 1e9, "frac": nbytes / (ms_eager * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def extras_report(hm, torch, stream, plan_b, Xb, Yb) -> dict:

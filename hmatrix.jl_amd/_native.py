@@ -17,8 +17,8 @@ STORAGE = {"f64": HM_F64, "f32": HM_F32, "bf16": HM_BF16, "bf16x2": HM_BF16X2}
 STATUS = {0: "HM_OK", -1: "HM_EINVAL", -2: "HM_EHIP", -3: "HM_ENOMEM", -4: "HM_EUNSUPPORTED"}
 INFO_FIELDS = ("nrows", "ncols", "ndense", "nrk", "dense_elems", "rk_elems", "storage",
                "device_bytes", "factor_bytes", "maxrank", "proj_chunk", "row_waves", "flags", "strip_shape",
-               "batched")
-HM_NO_TRANSPOSE, HM_NO_BATCHED = 1, 2
+               "batched", "fused", "wait_faults")
+HM_NO_TRANSPOSE, HM_NO_BATCHED, HM_ONE_LAUNCH = 1, 2, 4
 HM_STRIPS_AUTO, HM_STRIPS_NARROW, HM_STRIPS_WIDE = 0, 1, 2
 HM_OP_N, HM_OP_T = 0, 1
 

@@ -51,7 +51,7 @@ struct ProjItem {
     int32_t ld;
     int32_t rows;
     int32_t kc;       // 1 .. PROJ_KMAX
-    int32_t pad;
+    int32_t pad;      // member of a many-matrix plan (0 otherwise)
 };
 
 // Reduction item: t[tOff + j] = sum_c part[pOff + c*kc + j], with part the
@@ -72,7 +72,24 @@ struct TileDev {
     int64_t base;
     int32_t rows;
     int32_t c0, c1;
-    int32_t pad;
+    int32_t pad;      // member of a many-matrix plan (0 otherwise)
+};
+
+// One-launch single-right-hand-side product of a many-matrix plan (nmem
+// members).  Workgroup b of the launch counts its finished items on slot
+// b % FUSED_SLOTS (the XCD it runs on) of its member's counters; every
+// counter has a 256-byte line of its own.  Counters (lines), zero between
+// products: [items (projection + reduction) | row tiles] per (member, slot),
+// then one per member (tile slots complete).  need (uint32): items, then
+// projection items, then row tiles per (member, slot), then the number of
+// slots with row tiles per member.  redmem = member of each reduction item.
+constexpr int FUSED_SLOTS = 8;
+constexpr int FUSED_STRIDE = 64;
+struct Fused {
+    int nmem = 0;
+    uint32_t *counts = nullptr;
+    uint32_t *need = nullptr;
+    int32_t *redmem = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -188,6 +205,7 @@ struct Direction {
     int64_t partLen = 0;   // length of the chunk partials
     void *t = nullptr;     // scratch [t (tLen) | partials (partLen)]
     void *tm = nullptr;    // the same for RM right-hand sides at once, RM values per entry
+    Fused fmem;            // many-matrix plans: the one-launch product
     Batched bat;
 };
 
@@ -235,6 +253,7 @@ struct Plan {
     // ZERO_BYTES of zeros: inactive lanes load from here instead of taking a
     // branch, so a wavefront's loads all issue before its first wait.
     void *zeros = nullptr;
+    uint32_t *fault = nullptr;     // raised when a bounded wait of the one-launch product ran out
     Direction fwd, bwd;
     // adjoint: gradient items and the balance exponent of every rank column
     GradItem *grad = nullptr;

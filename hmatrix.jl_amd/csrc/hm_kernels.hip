@@ -74,13 +74,27 @@ __device__ __forceinline__ double bcast(double v, int src) {
 // unconditional and all of them issue before the first wait: a round covers
 // U 64-row strips of all KC columns (about 32 loads per lane), lanes stride
 // the rows, each lane keeps KC accumulators, one wave reduction per column.
-template <class St, int KC>
+// Device-scope stores and loads (coherent across XCDs) for values that one
+// workgroup of a launch writes and another reads in the same launch.
+template <class T>
+__device__ __forceinline__ void store_dev(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T load_dev(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class St, int KC, bool DEV = false>
 __device__ __forceinline__ void proj_item(const ProjItem &w, const void *__restrict__ fac,
                                           const void *__restrict__ fac_lo,
                                           const typename St::acc *__restrict__ x,
                                           typename St::acc *__restrict__ out, int lane) {
     using A = typename St::acc;
-    constexpr int U = KC >= 32 ? 1 : (32 / KC > PROJ_CHUNK / 64 ? PROJ_CHUNK / 64 : 32 / KC);
+    // The fused kernel (DEV) shares one register budget with the row tiles:
+    // eight loads per lane and column group there keeps it at 8 waves/SIMD.
+    constexpr int U0 = KC >= 32 ? 1 : (32 / KC > PROJ_CHUNK / 64 ? PROJ_CHUNK / 64 : 32 / KC);
+    constexpr int U = DEV && U0 > 8 / KC ? (8 / KC > 0 ? 8 / KC : 1) : U0;
     const int rows = w.rows;
     const int64_t ld = w.ld;
     const A *xs = x + w.xoff;
@@ -116,7 +130,10 @@ __device__ __forceinline__ void proj_item(const ProjItem &w, const void *__restr
         for (int j = 0; j < KC; ++j) acc[j] += __shfl_xor(acc[j], o, 64);
     if (lane == 0) {
 #pragma unroll
-        for (int j = 0; j < KC; ++j) out[w.out + j] = acc[j];
+        for (int j = 0; j < KC; ++j) {
+            if constexpr (DEV) store_dev(out + w.out + j, acc[j]);
+            else out[w.out + j] = acc[j];
+        }
     }
 }
 
@@ -183,6 +200,112 @@ __global__ __launch_bounds__(256) void k_reduce(const RedItem *__restrict__ item
     t[r.tOff + j] = s;
 }
 
+// Pass 2, one row tile (the body of k_rows and of the fused kernel below).
+// DEV: t is written earlier in the same launch; each wavefront issues
+// ready.start(lane) (the first readiness check), its first multipliers and
+// stream loads, then ready.finish(lane, started), which returns once the t
+// values the tile reads are complete, then reads t with device-scope loads.
+struct NoWait {
+    __device__ uint32_t start(int) const { return 0; }
+    __device__ void finish(int, uint32_t) const {}
+};
+
+template <class St, int NW, bool DEV, class Ready = NoWait>
+__device__ __forceinline__ void rows_tile(const TileDev &T, const int32_t *__restrict__ mult,
+                                          const void *__restrict__ stream, const void *__restrict__ stream_lo,
+                                          const typename St::acc *__restrict__ x,
+                                          const typename St::acc *__restrict__ t, typename St::acc *__restrict__ y,
+                                          typename St::acc alpha, typename St::acc beta, int beta_zero,
+                                          typename St::acc (*red)[64], Ready ready = Ready()) {
+    using A = typename St::acc;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int R = T.rows;
+    const bool act = lane < R;
+    const int ncol = T.c1 - T.c0;
+    const int per = (ncol + NW - 1) / NW;
+    const int cb = T.c0 + min(ncol, wave * per);
+    const int ce = T.c0 + min(ncol, (wave + 1) * per);
+    A a0 = A(0), a1 = A(0), a2 = A(0), a3 = A(0);
+    // Lanes past the tile's rows read row R-1 (a valid address) and their
+    // sums are never written: every load is unconditional.
+    const int lr = min(lane, R - 1);
+    if constexpr (DEV) {
+        // groups of eight columns; the last group of a chunk loads column
+        // nc-1 in place of the missing ones and skips them in the sums
+        bool first = true;
+        const uint32_t started = cb < ce ? ready.start(lane) : 0u;
+        for (int c = cb; c < ce; c += 64) {
+            const int nc = min(64, ce - c);
+            const int32_t id = mult[c + min(lane, nc - 1)];
+            const int64_t p0 = T.base + static_cast<int64_t>(c - T.c0) * R + lr;
+            A v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(min(u, nc - 1)) * R);
+            if (first) {
+                ready.finish(lane, started);
+                first = false;
+            }
+            const A mv = id >= 0 ? x[id] : load_dev(t + ~id);
+            for (int j = 0; j < nc; j += 8) {
+                if (j > 0) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        v[u] = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(min(j + u, nc - 1)) * R);
+                }
+                if (j + 8 <= nc) {
+                    a0 += v[0] * bcast(mv, j);
+                    a1 += v[1] * bcast(mv, j + 1);
+                    a2 += v[2] * bcast(mv, j + 2);
+                    a3 += v[3] * bcast(mv, j + 3);
+                    a0 += v[4] * bcast(mv, j + 4);
+                    a1 += v[5] * bcast(mv, j + 5);
+                    a2 += v[6] * bcast(mv, j + 6);
+                    a3 += v[7] * bcast(mv, j + 7);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (j + u < nc) a0 += v[u] * bcast(mv, j + u);
+                }
+            }
+        }
+    } else {
+        for (int c = cb; c < ce; c += 64) {
+            const int nc = min(64, ce - c);
+            const int32_t id = mult[c + min(lane, nc - 1)];
+            const A mv = *(id >= 0 ? x + id : t + ~id);
+            const int64_t p0 = T.base + static_cast<int64_t>(c - T.c0) * R + lr;
+            int j = 0;
+            for (; j + 8 <= nc; j += 8) {
+                A v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j + u) * R);
+                a0 += v[0] * bcast(mv, j);
+                a1 += v[1] * bcast(mv, j + 1);
+                a2 += v[2] * bcast(mv, j + 2);
+                a3 += v[3] * bcast(mv, j + 3);
+                a0 += v[4] * bcast(mv, j + 4);
+                a1 += v[5] * bcast(mv, j + 5);
+                a2 += v[6] * bcast(mv, j + 6);
+                a3 += v[7] * bcast(mv, j + 7);
+            }
+            for (; j < nc; ++j) {
+                const A v = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j) * R);
+                a0 += v * bcast(mv, j);
+            }
+        }
+    }
+    red[wave][lane] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    if (wave == 0 && act) {
+        A s = red[0][lane];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) s += red[w][lane];
+        const int64_t r = T.row0 + lane;
+        y[r] = beta_zero ? alpha * s : beta * y[r] + alpha * s;
+    }
+}
+
 // Pass 2: row owner over the tile's column stream.  One workgroup per tile;
 // its NW wavefronts take contiguous parts of the columns, lane = row.
 // Multipliers are fetched 64 at a time (one per lane) and broadcast with
@@ -199,52 +322,7 @@ __global__ __launch_bounds__(64 * NW) void k_rows(const TileDev *__restrict__ ti
                                               int beta_zero) {
     using A = typename St::acc;
     __shared__ A red[NW][64];
-    const TileDev T = tiles[blockIdx.x];
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int R = T.rows;
-    const bool act = lane < R;
-    const int ncol = T.c1 - T.c0;
-    const int per = (ncol + NW - 1) / NW;
-    const int cb = T.c0 + min(ncol, wave * per);
-    const int ce = T.c0 + min(ncol, (wave + 1) * per);
-    A a0 = A(0), a1 = A(0), a2 = A(0), a3 = A(0);
-    // Lanes past the tile's rows read row R-1 (a valid address) and their
-    // sums are never written: every load is unconditional.
-    const int lr = min(lane, R - 1);
-    for (int c = cb; c < ce; c += 64) {
-        const int nc = min(64, ce - c);
-        const int32_t id = mult[c + min(lane, nc - 1)];
-        const A mv = *(id >= 0 ? x + id : t + ~id);
-        const int64_t p0 = T.base + static_cast<int64_t>(c - T.c0) * R + lr;
-        int j = 0;
-        for (; j + 8 <= nc; j += 8) {
-            A v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j + u) * R);
-            a0 += v[0] * bcast(mv, j);
-            a1 += v[1] * bcast(mv, j + 1);
-            a2 += v[2] * bcast(mv, j + 2);
-            a3 += v[3] * bcast(mv, j + 3);
-            a0 += v[4] * bcast(mv, j + 4);
-            a1 += v[5] * bcast(mv, j + 5);
-            a2 += v[6] * bcast(mv, j + 6);
-            a3 += v[7] * bcast(mv, j + 7);
-        }
-        for (; j < nc; ++j) {
-            const A v = St::ld(stream, stream_lo, p0 + static_cast<int64_t>(j) * R);
-            a0 += v * bcast(mv, j);
-        }
-    }
-    red[wave][lane] = (a0 + a1) + (a2 + a3);
-    __syncthreads();
-    if (wave == 0 && act) {
-        A s = red[0][lane];
-#pragma unroll
-        for (int w = 1; w < NW; ++w) s += red[w][lane];
-        const int64_t r = T.row0 + lane;
-        y[r] = beta_zero ? alpha * s : beta * y[r] + alpha * s;
-    }
+    rows_tile<St, NW, false>(tiles[blockIdx.x], mult, stream, stream_lo, x, t, y, alpha, beta, beta_zero, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -526,6 +604,197 @@ __global__ __launch_bounds__(256) void k_loss_final(const double *__restrict__ p
     if (threadIdx.x == 0) *out = 0.5 * sh[0];
 }
 
+// ---------------------------------------------------------------------------
+// One launch for a single-right-hand-side product of a many-matrix plan:
+// the workgroups take, in this order, the projection items (NW per
+// workgroup), the reduction items (64 NW / PROJ_KMAX per workgroup) and the
+// row tiles, with the members in decreasing order of bytes (largest first)
+// inside each group.  A member's reductions wait until its projection items
+// are done and its row tiles until its projection and reduction items are
+// done, by counters in device memory; every item a workgroup waits for has a
+// lower workgroup index, so with workgroups dispatched in index order the
+// waits end.  A wait is bounded all the same (FUSED_SPIN polls): past it the
+// workgroup goes on and raises *fault, so that the kernel always ends.  The
+// member's last row tile clears its counters for the next product.  t and
+// the partial sums cross workgroups (and XCDs) with device-scope stores and
+// loads; each wavefront waits for its stores to complete, then one thread
+// per workgroup adds the workgroup's items to the counters (one add per
+// member).  Each member's counters are split into FUSED_SLOTS slots, one per
+// XCD, each on its own 256-byte line, and a waiter polls all slots at once
+// (one lane each): one line per member took one add per item from every
+// XCD and serialised the whole launch on a few lines.
+constexpr int FUSED_SPIN = 1 << 22;
+
+// Lanes 0 .. FUSED_SLOTS-1 each wait until counter slot `lane` (lines of
+// FUSED_STRIDE words from cnt) reaches need[lane]; the other lanes pass.
+// start() issues the first check, finish() completes it and polls on.
+struct SlotWait {
+    const uint32_t *cnt;
+    const uint32_t *need;
+    uint32_t *fault;
+    __device__ uint32_t start(int lane) const { return lane < FUSED_SLOTS ? load_dev(cnt + lane * FUSED_STRIDE) : 0u; }
+    __device__ void finish(int lane, uint32_t seen) const {
+        if (lane >= FUSED_SLOTS) return;
+        const uint32_t target = need[lane];
+        if (seen >= target) return;
+        const uint32_t *c = cnt + lane * FUSED_STRIDE;
+        for (int it = 0; it < FUSED_SPIN; ++it) {
+            __builtin_amdgcn_s_sleep(2);
+            if (load_dev(c) >= target) return;
+        }
+        store_dev(fault, 1u);
+    }
+    __device__ void wait(int lane) const { finish(lane, start(lane)); }
+};
+
+// One thread: for each run of equal members in mem[0 .. n) (-1: no item),
+// add the run's length to that member's item counter on `slot`.
+__device__ __forceinline__ void count_runs(uint32_t *counts, const int *mem, int n, int slot) {
+    for (int i = 0; i < n;) {
+        const int m = mem[i];
+        int j = i + 1;
+        while (j < n && mem[j] == m) ++j;
+        if (m >= 0)
+            __hip_atomic_fetch_add(counts + (m * FUSED_SLOTS + slot) * FUSED_STRIDE, static_cast<uint32_t>(j - i),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        i = j;
+    }
+}
+
+// Counter lines and need entries are laid out as described with struct Fused.
+template <class St, int NW>
+__global__ __launch_bounds__(64 * NW) void k_fused(const ProjItem *__restrict__ proj, int64_t nproj,
+                                               const RedItem *__restrict__ redi, const int32_t *__restrict__ redmem,
+                                               int64_t nred, const TileDev *__restrict__ tiles,
+                                               const int32_t *__restrict__ mult, const void *__restrict__ stream,
+                                               const void *__restrict__ stream_lo, const void *__restrict__ fac,
+                                               const void *__restrict__ fac_lo,
+                                               const typename St::acc *__restrict__ x,
+                                               typename St::acc *__restrict__ t, int64_t tLen,
+                                               typename St::acc *__restrict__ y, typename St::acc alpha,
+                                               typename St::acc beta, int beta_zero, uint32_t *__restrict__ counts,
+                                               const uint32_t *__restrict__ need, int nmem,
+                                               uint32_t *__restrict__ fault) {
+    using A = typename St::acc;
+    constexpr int S = FUSED_SLOTS;
+    constexpr int RPW = 64 * NW / PROJ_KMAX;               // reduction items per workgroup
+    __shared__ A red[NW][64];
+    __shared__ int wm[RPW > NW ? RPW : NW];
+    const int64_t npw = (nproj + NW - 1) / NW;
+    const int64_t nrw = (nred + RPW - 1) / RPW;
+    const int64_t b = blockIdx.x;
+    const int slot = static_cast<int>(b % S);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int64_t MS = static_cast<int64_t>(nmem) * S;
+    if (b < npw) {                                         // projection items, one per wavefront
+        const int64_t it = b * NW + wave;
+        int m = -1;
+        if (it < nproj) {
+            const ProjItem w = proj[it];
+            static_assert(PROJ_KMAX == 16, "one case per rank-column count");
+            switch (w.kc) {
+#define HM_PCASE(K) case K: proj_item<St, K, true>(w, fac, fac_lo, x, t, lane); break;
+                HM_PCASE(1) HM_PCASE(2) HM_PCASE(3) HM_PCASE(4) HM_PCASE(5) HM_PCASE(6) HM_PCASE(7) HM_PCASE(8)
+                HM_PCASE(9) HM_PCASE(10) HM_PCASE(11) HM_PCASE(12) HM_PCASE(13) HM_PCASE(14) HM_PCASE(15)
+                HM_PCASE(16)
+#undef HM_PCASE
+                default: break;
+            }
+            m = w.pad;
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wavefront's stores are done
+        if (lane == 0) wm[wave] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) count_runs(counts, wm, NW, slot);
+        return;
+    }
+    if (b < npw + nrw) {                                   // reduction items, PROJ_KMAX lanes each
+        const int g = static_cast<int>(threadIdx.x / PROJ_KMAX);
+        const int j = static_cast<int>(threadIdx.x % PROJ_KMAX);
+        const int64_t it = (b - npw) * RPW + g;
+        int m = -1;
+        if (it < nred) {
+            const RedItem r = redi[it];
+            m = redmem[it];
+            SlotWait{counts + m * S * FUSED_STRIDE, need + MS + m * S, fault}.wait(j);   // its projections
+            if (j < r.kc) {
+                const A *src = t + tLen + r.pOff + j;
+                A sum = A(0);
+                for (int c0 = 0; c0 < r.nchunk; c0 += RED_ROUND) {
+                    A v[RED_ROUND];
+#pragma unroll
+                    for (int u = 0; u < RED_ROUND; ++u) {
+                        const int c = c0 + u < r.nchunk ? c0 + u : 0;
+                        v[u] = load_dev(src + static_cast<int64_t>(c) * r.kc);
+                    }
+#pragma unroll
+                    for (int u = 0; u < RED_ROUND; ++u)
+                        if (c0 + u < r.nchunk) sum += v[u];
+                }
+                store_dev(t + r.tOff + j, sum);
+            }
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (j == 0) wm[g] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) count_runs(counts, wm, RPW, slot);
+        return;
+    }
+    const TileDev T = tiles[b - npw - nrw];               // row tile
+    const int m = T.pad;
+    rows_tile<St, NW, true>(T, mult, stream, stream_lo, x, t, y, alpha, beta, beta_zero, red,
+                            SlotWait{counts + m * S * FUSED_STRIDE, need + m * S, fault});
+    if (threadIdx.x == 0) {
+        // the member's last row tile (last on its slot, then last slot)
+        // clears the member's counters for the next product
+        uint32_t *tc = counts + (MS + m * S + slot) * FUSED_STRIDE;
+        const uint32_t p1 = __hip_atomic_fetch_add(tc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p1 + 1 == need[2 * MS + m * S + slot]) {
+            uint32_t *mc = counts + (2 * MS + m) * FUSED_STRIDE;
+            const uint32_t p2 = __hip_atomic_fetch_add(mc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (p2 + 1 == need[3 * MS + m]) {
+                for (int q = 0; q < S; ++q) {
+                    store_dev(counts + (m * S + q) * FUSED_STRIDE, 0u);
+                    store_dev(counts + (MS + m * S + q) * FUSED_STRIDE, 0u);
+                }
+                store_dev(mc, 0u);
+            }
+        }
+    }
+}
+
+template <class St>
+static hipError_t run_fused(const Plan &p, const Direction &d, const void *xv, void *yv, double alpha,
+                            double beta, hipStream_t s, hipEvent_t *ev, void *scratch) {
+    using A = typename St::acc;
+    const A *x = static_cast<const A *>(xv);
+    A *y = static_cast<A *>(yv);
+    A *t = static_cast<A *>(scratch ? scratch : d.t);
+    if (ev) {
+        (void)hipEventRecord(ev[0], s);
+        (void)hipEventRecord(ev[1], s);
+    }
+    const int NW = p.row_waves;
+    const int64_t npw = (d.nproj + NW - 1) / NW;
+    const int64_t rpw = 64 * NW / PROJ_KMAX;
+    const int64_t nrw = (d.nred + rpw - 1) / rpw;
+    const dim3 grid(static_cast<unsigned>(npw + nrw + d.ntiles));
+    const A al = static_cast<A>(alpha), be = static_cast<A>(beta);
+    const int bz = beta == 0.0 ? 1 : 0;
+#define HM_FUSED(W)                                                                                             \
+    hipLaunchKernelGGL((k_fused<St, W>), grid, dim3(64 * W), 0, s, d.proj, d.nproj, d.red, d.fmem.redmem, d.nred, \
+                       d.tiles, d.mult, d.stream, d.stream_lo, p.fac, p.fac_lo, x, t, d.tLen, y, al, be, bz,      \
+                       d.fmem.counts, d.fmem.need, d.fmem.nmem, p.fault)
+    if (NW == 2) HM_FUSED(2);
+    else if (NW == 8) HM_FUSED(8);
+    else HM_FUSED(4);
+#undef HM_FUSED
+    const hipError_t e = hipGetLastError();
+    if (ev) (void)hipEventRecord(ev[2], s);
+    return e;
+}
+
 template <class A>
 static hipError_t run_grads(const Plan &p, const void *x, const void *res, void *grad,
                             bool want_loss, hipStream_t s) {
@@ -661,6 +930,15 @@ hipError_t launch_matvec_m(const Plan &p, const Direction &d, int ncol, const vo
 
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y, double alpha,
                          double beta, hipStream_t s, hipEvent_t *ev, void *scratch) {
+    if (d.fmem.nmem > 0 && d.ntiles > 0) {
+        switch (p.storage) {
+            case 0: return run_fused<StF64>(p, d, x, y, alpha, beta, s, ev, scratch);
+            case 1: return run_fused<StF32>(p, d, x, y, alpha, beta, s, ev, scratch);
+            case 2: return run_fused<StBF16>(p, d, x, y, alpha, beta, s, ev, scratch);
+            case 3: return run_fused<StBF16x2>(p, d, x, y, alpha, beta, s, ev, scratch);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (p.storage) {
         case 0: return run<StF64>(p, d, x, y, alpha, beta, s, ev, scratch);
         case 1: return run<StF32>(p, d, x, y, alpha, beta, s, ev, scratch);

@@ -122,9 +122,12 @@ void pack_storage(int st, const double *src, int64_t n, std::vector<uint8_t> &hi
 
 
 // Row tiles: cut at every leaf row boundary, then into <= TILE_ROWS pieces.
-std::vector<std::pair<int64_t, int32_t>> make_tiles(int64_t nrows, const std::vector<OB> &obs) {
-    std::vector<int64_t> cuts;
-    cuts.reserve(2 * obs.size() + 2);
+// Row tiles of at most TILE_ROWS rows, cut at every leaf's first and end
+// row and at the rows in `extra` (member boundaries of a many-matrix plan).
+std::vector<std::pair<int64_t, int32_t>> make_tiles(int64_t nrows, const std::vector<OB> &obs,
+                                                    const std::vector<int64_t> &extra = {}) {
+    std::vector<int64_t> cuts(extra.begin(), extra.end());
+    cuts.reserve(2 * obs.size() + 2 + extra.size());
     cuts.push_back(0);
     cuts.push_back(nrows);
     for (const OB &o : obs) {
@@ -565,8 +568,12 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
 // Single-right-hand-side layout of one direction: projection items over the
 // factor arena `fac` (uploaded once per plan) and the row tiles' column
 // streams built from `fac` (low-rank A) and `dense` (dense leaves).
+// mrow (many-matrix plans, else empty): first row of each member in this
+// direction, and nrows last; the direction then gets the one-launch layout
+// (items and tiles ordered by member, largest members first).
 void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
-                     const std::vector<OB> &obs, const double *fac, const double *dense) {
+                     const std::vector<OB> &obs, const double *fac, const double *dense,
+                     const std::vector<int64_t> &mrow) {
     using namespace hm;
     d.nrows = nrows;
     d.ncols = ncols;
@@ -574,6 +581,7 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
     std::vector<BlkDev> blk(obs.size());
     std::vector<ProjItem> proj;
     std::vector<RedItem> red;
+    std::vector<int64_t> pleaf, rleaf;     // leaf of each projection / reduction item
     // t offsets
     int64_t tLen = 0;
     for (int64_t b = 0; b < nb; ++b) {
@@ -608,6 +616,7 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
                 pOff = partLen;
                 partLen += nchunk * kc;
                 red.push_back(RedItem{pOff, tDst, static_cast<int32_t>(nchunk), kc});
+                rleaf.push_back(b);
             }
             for (int64_t c = 0; c < nchunk; ++c) {
                 ProjItem it{};
@@ -619,11 +628,12 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
                 it.kc = kc;
                 it.out = nchunk > 1 ? tLen + pOff + c * kc : tDst;
                 proj.push_back(it);
+                pleaf.push_back(b);
             }
         }
     }
     // row tiles and their column streams (tile-major, leaves in table order)
-    const auto tl = make_tiles(nrows, obs);
+    auto tl = make_tiles(nrows, obs, mrow);
     std::vector<int64_t> tileStart(tl.size());
     for (size_t i = 0; i < tl.size(); ++i) tileStart[i] = tl[i].first;
     std::vector<std::vector<std::pair<int32_t, int32_t>>> per(tl.size());   // (leaf, row offset)
@@ -666,6 +676,93 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
         T.c1 = static_cast<int32_t>(mult.size());
     }
     if (tLen >= (int64_t(1) << 31)) throw std::length_error("rank total exceeds 2^31");
+    std::vector<int32_t> redmem;
+    std::vector<uint32_t> need;
+    const int nmem = mrow.size() > 2 ? static_cast<int>(mrow.size() - 1) : 0;
+    if (nmem > 0) {
+        // member holding `row`: the last member that starts at or before it
+        auto member_of = [&](int64_t row) {
+            const int64_t q = std::upper_bound(mrow.begin(), mrow.end(), row) - mrow.begin() - 1;
+            return static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(q, 0), nmem - 1));
+        };
+        // bytes per member (stored elements); members in decreasing order
+        std::vector<double> bytes(static_cast<size_t>(nmem), 0.0);
+        for (const OB &o : obs) {
+            if (!contributes(o)) continue;
+            bytes[member_of(o.row0)] += static_cast<double>(o.kind == KIND_DENSE ? o.m * o.n : (o.m + o.n) * o.k);
+        }
+        std::vector<int32_t> order(static_cast<size_t>(nmem)), rank(static_cast<size_t>(nmem));
+        for (int i = 0; i < nmem; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return bytes[a] > bytes[b]; });
+        for (int i = 0; i < nmem; ++i) rank[order[i]] = i;
+        // projection items: member of their leaf (the leaf's first row); by
+        // member rank, then larger items first
+        std::vector<int32_t> pmem(proj.size());
+        for (size_t i = 0; i < proj.size(); ++i) pmem[i] = member_of(obs[pleaf[i]].row0);
+        std::vector<size_t> pi(proj.size());
+        for (size_t i = 0; i < pi.size(); ++i) pi[i] = i;
+        std::stable_sort(pi.begin(), pi.end(), [&](size_t a, size_t b) {
+            if (rank[pmem[a]] != rank[pmem[b]]) return rank[pmem[a]] < rank[pmem[b]];
+            return int64_t(proj[a].rows) * proj[a].kc > int64_t(proj[b].rows) * proj[b].kc;
+        });
+        std::vector<ProjItem> proj2(proj.size());
+        for (size_t i = 0; i < pi.size(); ++i) {
+            proj2[i] = proj[pi[i]];
+            proj2[i].pad = pmem[pi[i]];
+        }
+        proj.swap(proj2);
+        // reduction items: member of their leaf
+        {
+            redmem.resize(red.size());
+            for (size_t i = 0; i < red.size(); ++i) redmem[i] = member_of(obs[rleaf[i]].row0);
+            std::vector<size_t> ri(red.size());
+            for (size_t i = 0; i < ri.size(); ++i) ri[i] = i;
+            std::stable_sort(ri.begin(), ri.end(), [&](size_t a, size_t b) { return rank[redmem[a]] < rank[redmem[b]]; });
+            std::vector<RedItem> red2(red.size());
+            std::vector<int32_t> rm2(red.size());
+            for (size_t i = 0; i < ri.size(); ++i) {
+                red2[i] = red[ri[i]];
+                rm2[i] = redmem[ri[i]];
+            }
+            red.swap(red2);
+            redmem.swap(rm2);
+        }
+        // row tiles: by member rank, then wider streams first
+        std::vector<size_t> ti(tiles.size());
+        for (size_t i = 0; i < ti.size(); ++i) ti[i] = i;
+        std::vector<int32_t> tmem(tiles.size());
+        for (size_t i = 0; i < tiles.size(); ++i) tmem[i] = member_of(tiles[i].row0);
+        std::stable_sort(ti.begin(), ti.end(), [&](size_t a, size_t b) {
+            if (rank[tmem[a]] != rank[tmem[b]]) return rank[tmem[a]] < rank[tmem[b]];
+            return int64_t(tiles[a].c1 - tiles[a].c0) * tiles[a].rows > int64_t(tiles[b].c1 - tiles[b].c0) * tiles[b].rows;
+        });
+        std::vector<TileDev> tiles2(tiles.size());
+        for (size_t i = 0; i < ti.size(); ++i) {
+            tiles2[i] = tiles[ti[i]];
+            tiles2[i].pad = tmem[ti[i]];
+        }
+        tiles.swap(tiles2);
+        // slot of each item: the launch's workgroup index mod FUSED_SLOTS
+        // (NW projection items, then 64*NW/PROJ_KMAX reduction items, then
+        // one row tile per workgroup)
+        constexpr int S = FUSED_SLOTS;
+        const int64_t NW = p.row_waves, RPW = 64 * NW / PROJ_KMAX;
+        const int64_t npw = (static_cast<int64_t>(proj.size()) + NW - 1) / NW;
+        const int64_t nrw = (static_cast<int64_t>(red.size()) + RPW - 1) / RPW;
+        const size_t MS = static_cast<size_t>(nmem) * S;
+        need.assign(3 * MS + static_cast<size_t>(nmem), 0u);
+        for (size_t i = 0; i < proj.size(); ++i) {
+            const size_t q = static_cast<size_t>(proj[i].pad) * S + static_cast<size_t>((i / NW) % S);
+            ++need[q];
+            ++need[MS + q];
+        }
+        for (size_t i = 0; i < red.size(); ++i)
+            ++need[static_cast<size_t>(redmem[i]) * S + static_cast<size_t>((npw + i / RPW) % S)];
+        for (size_t i = 0; i < tiles.size(); ++i)
+            ++need[2 * MS + static_cast<size_t>(tiles[i].pad) * S + static_cast<size_t>((npw + nrw + i) % S)];
+        for (int m = 0; m < nmem; ++m)
+            for (int q = 0; q < S; ++q) need[3 * MS + m] += need[2 * MS + m * S + q] > 0 ? 1u : 0u;
+    }
     std::vector<uint8_t> shi, slo;
     pack_storage(p.storage, col.data(), static_cast<int64_t>(col.size()), shi, slo);
     d.stream = shi.empty() ? nullptr : static_cast<void *>(dev_upload(p, shi));
@@ -684,6 +781,14 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
     d.partLen = partLen;
     d.t = dev_alloc(p, static_cast<size_t>(tLen + partLen) * acc_size(p.storage));
     d.tm = dev_alloc(p, static_cast<size_t>(tLen + partLen) * MULTI_RHS * acc_size(p.storage));
+    if (nmem > 0) {
+        d.fmem.nmem = nmem;
+        d.fmem.need = dev_upload(p, need);
+        d.fmem.redmem = red.empty() ? nullptr : dev_upload(p, redmem);
+        const size_t cbytes = sizeof(uint32_t) * FUSED_STRIDE * (2 * FUSED_SLOTS + 1) * static_cast<size_t>(nmem);
+        d.fmem.counts = static_cast<uint32_t *>(dev_alloc(p, cbytes));
+        hip_check(hipMemset(d.fmem.counts, 0, cbytes), "hipMemset");
+    }
 }
 
 struct DeviceGuard {
@@ -829,23 +934,27 @@ int hm_plan_create(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t 
 namespace {
 int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *blocks,
                 const double *data, int64_t ndata, int storage, const hm_config *cfg,
-                bool check_tiling, hm_plan_t *plan);
+                bool check_tiling, const std::vector<int64_t> &mrow, const std::vector<int64_t> &mcol,
+                hm_plan_t *plan);
 }
 
 extern "C" int hm_plan_create_ex(int64_t nrows, int64_t ncols, int64_t nblocks,
                                  const int64_t *blocks, const double *data, int64_t ndata,
                                  int storage, const hm_config *cfg, hm_plan_t *plan) {
     g_err.clear();
-    return create_impl(nrows, ncols, nblocks, blocks, data, ndata, storage, cfg, true, plan);
+    return create_impl(nrows, ncols, nblocks, blocks, data, ndata, storage, cfg, true, {}, {}, plan);
 }
 
 namespace {
 // check_tiling: the leaves must tile the matrix exactly once (a plan made by
 // hm_plan_create_batch checks each member instead; the off-diagonal part of
-// its block-diagonal matrix is zero and has no leaves).
+// its block-diagonal matrix is zero and has no leaves).  mrow / mcol: first
+// row / column of each member of such a plan and the total last (empty
+// otherwise).
 int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *blocks,
                 const double *data, int64_t ndata, int storage, const hm_config *cfg,
-                bool check_tiling, hm_plan_t *plan) {
+                bool check_tiling, const std::vector<int64_t> &mrow, const std::vector<int64_t> &mcol,
+                hm_plan_t *plan) {
     using namespace hm;
     if (!plan) return fail(HM_EINVAL, "plan output pointer is NULL");
     *plan = nullptr;
@@ -859,7 +968,7 @@ int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
         return fail(HM_EINVAL, "proj_chunk must be a multiple of 64 in [64, 65536]");
     if (c.row_waves != 2 && c.row_waves != 4 && c.row_waves != 8)
         return fail(HM_EINVAL, "row_waves must be 2, 4 or 8");
-    if (c.flags & ~(HM_NO_TRANSPOSE | HM_NO_BATCHED)) return fail(HM_EINVAL, "unknown flags");
+    if (c.flags & ~(HM_NO_TRANSPOSE | HM_NO_BATCHED | HM_ONE_LAUNCH)) return fail(HM_EINVAL, "unknown flags");
     if (c.strip_shape < HM_STRIPS_AUTO || c.strip_shape > HM_STRIPS_WIDE)
         return fail(HM_EINVAL, "strip_shape must be 0 (auto), 1 (narrow) or 2 (wide)");
     if (nrows < 0 || ncols < 0 || nblocks < 0 || ndata < 0)
@@ -979,8 +1088,15 @@ int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
             }
             bw[b] = t;
         }
-        build_direction(*p, p->fwd, nrows, ncols, fw, rkv.data(), data);
-        if (!(p->flags & HM_NO_TRANSPOSE)) build_direction(*p, p->bwd, ncols, nrows, bw, rkv.data(), dT.data());
+        const bool fused = mrow.size() > 2 && (c.flags & HM_ONE_LAUNCH);
+        if (fused) {
+            p->fault = static_cast<uint32_t *>(dev_alloc(*p, sizeof(uint32_t)));
+            hip_check(hipMemset(p->fault, 0, sizeof(uint32_t)), "hipMemset");
+        }
+        const std::vector<int64_t> none;
+        build_direction(*p, p->fwd, nrows, ncols, fw, rkv.data(), data, fused ? mrow : none);
+        if (!(p->flags & HM_NO_TRANSPOSE))
+            build_direction(*p, p->bwd, ncols, nrows, bw, rkv.data(), dT.data(), fused ? mcol : none);
         // adjoint gradient items over the caller's own arena offsets
         {
             // items of up to GRAD_STRIPS 64-row strips and about 64 stores per lane
@@ -1059,6 +1175,12 @@ int hm_plan_info(hm_plan_t plan, int64_t *out, int nout) {
     const hm::Plan *p = reinterpret_cast<const hm::Plan *>(plan);
     if (!p || (!out && nout > 0)) return fail(HM_EINVAL, "NULL argument");
     const int64_t es = static_cast<int64_t>(storage_size(p->storage)) * (p->storage == HM_BF16X2 ? 2 : 1);
+    uint32_t faults = 0;
+    if (p->fault) {
+        DeviceGuard g(p->device);
+        const hipError_t e = hipMemcpy(&faults, p->fault, sizeof faults, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail(HM_EHIP, std::string("hm_plan_info: ") + hipGetErrorString(e));
+    }
     const int64_t v[HM_INFO_NFIELDS] = {p->nrows,
                                         p->ncols,
                                         p->ndense,
@@ -1074,7 +1196,9 @@ int hm_plan_info(hm_plan_t plan, int64_t *out, int nout) {
                                         p->flags,
                                         strip_shape_in_use(*p),
                                         (p->fwd.bat.ready &&
-                                         ((p->flags & HM_NO_TRANSPOSE) || p->bwd.bat.ready)) ? 1 : 0};
+                                         ((p->flags & HM_NO_TRANSPOSE) || p->bwd.bat.ready)) ? 1 : 0,
+                                        p->fwd.fmem.nmem > 0 ? 1 : 0,
+                                        static_cast<int64_t>(faults)};
     for (int i = 0; i < nout && i < HM_INFO_NFIELDS; ++i) out[i] = v[i];
     return HM_OK;
 }
@@ -1301,7 +1425,10 @@ int hm_plan_create_batch(int64_t nmat, const int64_t *nrows, const int64_t *ncol
         return fail(HM_ENOMEM, "host allocation failed");
     }
     int64_t r0 = 0, c0 = 0, q = 0, d0 = 0;
+    std::vector<int64_t> mrow, mcol;
     for (int64_t i = 0; i < nmat; ++i) {
+        mrow.push_back(r0);
+        mcol.push_back(c0);
         for (int64_t b = 0; b < nblocks[i]; ++b, ++q) {
             const int64_t *src = blocks[i] + 8 * b;
             int64_t *dst = bt.data() + 8 * q;
@@ -1322,7 +1449,9 @@ int hm_plan_create_batch(int64_t nmat, const int64_t *nrows, const int64_t *ncol
         c0 += ncols[i];
         d0 += ndata[i];
     }
-    return create_impl(R, C, NB_, bt.data(), dt.data(), ND, storage, cfg, false, plan);
+    mrow.push_back(R);
+    mcol.push_back(C);
+    return create_impl(R, C, NB_, bt.data(), dt.data(), ND, storage, cfg, false, mrow, mcol, plan);
 }
 
 struct hm_graph_s {

@@ -45,7 +45,8 @@ class DeviceHmat:
 
         ``config`` (optional) sets ``hm_config`` fields: ``proj_chunk``,
         ``row_waves``, ``no_transpose``, ``no_batched``, ``strip_shape``
-        (0 auto, 1 narrow, 2 wide).
+        (0 auto, 1 narrow, 2 wide), ``one_launch`` (many-matrix plans: one
+        launch per single-right-hand-side product instead of three).
         """
         torch = _torch()
         if dtype not in _native.STORAGE:
@@ -59,7 +60,8 @@ class DeviceHmat:
         self.device = torch.device("cuda", torch.cuda.current_device())
         cfg = _native.HmConfig(int(self.config.get("proj_chunk", 0)), int(self.config.get("row_waves", 0)),
                                (_native.HM_NO_TRANSPOSE if self.config.get("no_transpose") else 0)
-                               | (_native.HM_NO_BATCHED if self.config.get("no_batched") else 0),
+                               | (_native.HM_NO_BATCHED if self.config.get("no_batched") else 0)
+                               | (_native.HM_ONE_LAUNCH if self.config.get("one_launch") else 0),
                                int(self.config.get("strip_shape", 0)))
         h = ctypes.c_void_p()
         if _batch is None:

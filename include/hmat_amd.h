@@ -74,13 +74,26 @@ enum hm_info_field {
                                 columns, or slice offsets past 2^32), and
                                 hm_matmat runs panels of 8 columns on the
                                 single-right-hand-side layout instead          */
+    HM_INFO_FUSED,           /* 1: a single-right-hand-side product of this plan
+                                (hm_plan_create_batch with HM_ONE_LAUNCH) runs
+                                as one launch whose row tiles wait on
+                                per-matrix counters for the projections they
+                                read                                          */
+    HM_INFO_WAIT_FAULTS,     /* nonzero: a bounded wait of that one launch ran
+                                out (products since then are unreliable; the
+                                plan must be recreated). Reading this field
+                                synchronises the device                       */
     HM_INFO_NFIELDS
 };
 
 /* hm_config.flags */
 enum hm_flags {
     HM_NO_TRANSPOSE = 1,     /* do not build the H' structures (halves memory)   */
-    HM_NO_BATCHED = 2        /* do not build the multi-right-hand-side layout    */
+    HM_NO_BATCHED = 2,       /* do not build the multi-right-hand-side layout    */
+    HM_ONE_LAUNCH = 4        /* plans of hm_plan_create_batch: run a single-
+                                right-hand-side product as one launch (row
+                                tiles wait on per-matrix counters) instead of
+                                three (projection, reduction, row tiles)       */
 };
 
 /* hm_config.strip_shape: workgroups of the multi-right-hand-side row pass */

@@ -14,15 +14,52 @@ def members(hm):
 
 
 def test_batch_one_launch_matches_oracle(hm, orc, members):
+    """HM_ONE_LAUNCH: the single-right-hand-side product of a batch runs as
+    one launch (projections, reductions and row tiles ordered largest member
+    first, row tiles waiting on per-member counters); every member equals the oracle,
+    for H and H', with alpha/beta, over products in a row (the counters
+    reset themselves), and no bounded wait ran out."""
+    import torch
+    d = hm.DeviceHmat.batch(members, "f32", {"one_launch": True})
+    assert d.info()["fused"] == 1
+    rng = np.random.default_rng(10)
+    for rep in range(3):
+        xs = [rng.standard_normal(m.ncols) for m in members]
+        x = torch.from_numpy(np.concatenate(xs)).float().cuda()
+        y = d.matvec(x).cpu().numpy()
+        for i, m in enumerate(members):
+            yi = y[d.row_offsets[i]:d.row_offsets[i + 1]]
+            assert relerr(yi, orc.matvec(m, xs[i])) <= 2e-6, (rep, i)
+    # H' over the same members, and y = 2 H x - 0.5 y
+    zs = [rng.standard_normal(m.nrows) for m in members]
+    z = torch.from_numpy(np.concatenate(zs)).float().cuda()
+    w = d.matvec(z, transpose=True).cpu().numpy()
+    for i, m in enumerate(members):
+        assert relerr(w[d.col_offsets[i]:d.col_offsets[i + 1]], orc.matvec_t(m, zs[i])) <= 2e-6, i
+    y0 = [rng.standard_normal(m.nrows) for m in members]
+    yd = torch.from_numpy(np.concatenate(y0)).float().cuda()
+    d.matvec(x, yd, alpha=2.0, beta=-0.5)
+    yv = yd.cpu().numpy()
+    for i, m in enumerate(members):
+        ref = 2.0 * orc.matvec(m, xs[i]) - 0.5 * y0[i]
+        assert relerr(yv[d.row_offsets[i]:d.row_offsets[i + 1]], ref) <= 4e-6, i
+    assert d.info()["wait_faults"] == 0
+
+
+def test_batch_three_launch_form(hm, orc, members):
+    """The default form: the same batch as projection, reduction and row
+    tile launches; every member equals the oracle, H and H'."""
     import torch
     d = hm.DeviceHmat.batch(members, "f32")
-    rng = np.random.default_rng(10)
+    assert d.info()["fused"] == 0
+    rng = np.random.default_rng(13)
     xs = [rng.standard_normal(m.ncols) for m in members]
-    x = torch.from_numpy(np.concatenate(xs)).float().cuda()
-    y = d.matvec(x).cpu().numpy()
+    y = d.matvec(torch.from_numpy(np.concatenate(xs)).float().cuda()).cpu().numpy()
+    zs = [rng.standard_normal(m.nrows) for m in members]
+    w = d.matvec(torch.from_numpy(np.concatenate(zs)).float().cuda(), transpose=True).cpu().numpy()
     for i, m in enumerate(members):
-        yi = y[d.row_offsets[i]:d.row_offsets[i + 1]]
-        assert relerr(yi, orc.matvec(m, xs[i])) <= 2e-6, i
+        assert relerr(y[d.row_offsets[i]:d.row_offsets[i + 1]], orc.matvec(m, xs[i])) <= 2e-6, i
+        assert relerr(w[d.col_offsets[i]:d.col_offsets[i + 1]], orc.matvec_t(m, zs[i])) <= 2e-6, i
 
 
 def test_batch_f64_bitwise_close(hm, orc, members):
@@ -35,9 +72,10 @@ def test_batch_f64_bitwise_close(hm, orc, members):
         assert relerr(y[d.row_offsets[i]:d.row_offsets[i + 1]], orc.matvec(m, xs[i])) <= 1e-13
 
 
-def test_graph_replay_matches_eager(hm, orc, members):
+@pytest.mark.parametrize("one_launch", [False, True])
+def test_graph_replay_matches_eager(hm, orc, members, one_launch):
     import torch
-    d = hm.DeviceHmat.batch(members, "f32")
+    d = hm.DeviceHmat.batch(members, "f32", {"one_launch": one_launch})
     x = torch.zeros(d.ncols, dtype=torch.float32, device="cuda")
     y = torch.zeros(d.nrows, dtype=torch.float32, device="cuda")
     g = d.graph(x, y)

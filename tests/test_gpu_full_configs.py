@@ -127,9 +127,10 @@ def c4(hm):
     return members
 
 
-def test_c4_every_member_vs_oracle(hm, orc, c4):
+@pytest.mark.parametrize("one_launch", [False, True])
+def test_c4_every_member_vs_oracle(hm, orc, c4, one_launch):
     import torch
-    d = hm.DeviceHmat.batch(c4, "f32", {"no_transpose": True, "no_batched": True})
+    d = hm.DeviceHmat.batch(c4, "f32", {"no_transpose": True, "no_batched": True, "one_launch": one_launch})
     rng = np.random.default_rng(12)
     xs = [rng.standard_normal(m.ncols) for m in c4]
     x = torch.from_numpy(np.concatenate(xs)).float().to(d.device)
@@ -139,13 +140,16 @@ def test_c4_every_member_vs_oracle(hm, orc, c4):
         e = relerr(y[d.row_offsets[i]:d.row_offsets[i + 1]], orc.matvec(m, xs[i]))
         worst = max(worst, e)
         assert e <= 2e-6, (i, m.nrows, e)
+    info = d.info()
+    assert info["fused"] == int(one_launch) and info["wait_faults"] == 0
     d.close()
     print(f"C4: worst member relerr {worst:.3e}", flush=True)
 
 
-def test_c4_graph_replay_bitwise(hm, c4):
+@pytest.mark.parametrize("one_launch", [False, True])
+def test_c4_graph_replay_bitwise(hm, c4, one_launch):
     import torch
-    d = hm.DeviceHmat.batch(c4, "f32", {"no_transpose": True, "no_batched": True})
+    d = hm.DeviceHmat.batch(c4, "f32", {"no_transpose": True, "no_batched": True, "one_launch": one_launch})
     x = torch.zeros(d.ncols, dtype=torch.float32, device=d.device)
     y = torch.zeros(d.nrows, dtype=torch.float32, device=d.device)
     g = d.graph(x, y)
@@ -158,4 +162,5 @@ def test_c4_graph_replay_bitwise(hm, c4):
         torch.cuda.synchronize()
         assert torch.equal(y, ref), rep
     g.close()
+    assert d.info()["wait_faults"] == 0
     d.close()
