@@ -46,6 +46,7 @@ SIGNATURES = {
     "hm_matvec_t": (ctypes.c_int, [_P, _P, _P, _D, _D, _P]),
     "hm_matmat_t": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _D, _D, _P]),
     "hm_adjoint": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, ctypes.POINTER(_D), _P]),
+    "hm_vjp": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "hm_graph_create": (ctypes.c_int, [_P, ctypes.c_int, _I64, _P, _I64, _P, _I64, _D, _D,
                                        ctypes.POINTER(_P)]),
     "hm_autotune": (ctypes.c_int, [_I64, _I64, _I64, _P, _P, _I64, ctypes.c_int, _I64, ctypes.c_int, _P,

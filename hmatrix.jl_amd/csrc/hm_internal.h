@@ -113,6 +113,9 @@ struct Plan {
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y,
                          double alpha, double beta, hipStream_t s, hipEvent_t *ev = nullptr,
                          void *scratch = nullptr);
+// The projection phase of a single-right-hand-side product alone: t = B' x
+// of every low-rank leaf of d into d.t.
+hipError_t launch_projection(const Plan &p, const Direction &d, const void *x, hipStream_t s);
 // Adjoint: gradients from res, x and the t vectors the two products left in
 // fwd.t (t = B' x) and bwd.t (s = A' res); optional loss into p.lossbuf.
 hipError_t launch_adjoint_grads(const Plan &p, const void *x, const void *res, void *grad,

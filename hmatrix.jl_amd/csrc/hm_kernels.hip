@@ -928,6 +928,33 @@ hipError_t launch_matvec_m(const Plan &p, const Direction &d, int ncol, const vo
     }
 }
 
+// The projection phase alone (t = B' x of every low-rank leaf into d.t).
+template <class St>
+static hipError_t run_proj(const Plan &p, const Direction &d, const void *xv, hipStream_t s) {
+    using A = typename St::acc;
+    const A *x = static_cast<const A *>(xv);
+    A *t = static_cast<A *>(d.t);
+    if (d.nproj == 0) return hipSuccess;
+    const unsigned g = static_cast<unsigned>((d.nproj + WAVES - 1) / WAVES);
+    hipLaunchKernelGGL(k_proj<St>, dim3(g), dim3(64 * WAVES), 0, s, d.proj, d.nproj, p.fac, p.fac_lo, x, t);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || d.nred == 0) return e;
+    const int64_t thr = d.nred * PROJ_KMAX;
+    hipLaunchKernelGGL(k_reduce<A>, dim3(static_cast<unsigned>((thr + 255) / 256)), dim3(256), 0, s, d.red, d.nred,
+                       static_cast<const A *>(t) + d.tLen, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_projection(const Plan &p, const Direction &d, const void *x, hipStream_t s) {
+    switch (p.storage) {
+        case 0: return run_proj<StF64>(p, d, x, s);
+        case 1: return run_proj<StF32>(p, d, x, s);
+        case 2: return run_proj<StBF16>(p, d, x, s);
+        case 3: return run_proj<StBF16x2>(p, d, x, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_matvec(const Plan &p, const Direction &d, const void *x, void *y, double alpha,
                          double beta, hipStream_t s, hipEvent_t *ev, void *scratch) {
     if (d.fmem.nmem > 0 && d.ntiles > 0) {

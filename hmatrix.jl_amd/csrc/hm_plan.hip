@@ -708,6 +708,29 @@ int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx
     return HM_OK;
 }
 
+int hm_vjp(hm_plan_t plan, const void *x, const void *g, void *dx, void *grad, void *stream) {
+    hm::Plan *p = reinterpret_cast<hm::Plan *>(plan);
+    if (!p) return fail(HM_EINVAL, "plan is NULL");
+    if (p->flags & HM_NO_TRANSPOSE) return fail(HM_EUNSUPPORTED, "plan was created with HM_NO_TRANSPOSE");
+    if ((p->ncols > 0 && (!x || !dx)) || (p->nrows > 0 && !g)) return fail(HM_EINVAL, "NULL vector");
+    const size_t es = acc_size(p->storage);
+    const int64_t m = p->nrows, n = p->ncols;
+    if (overlap(dx, n, x, n, es) || overlap(dx, n, g, m, es))
+        return fail(HM_EINVAL, "dx must not overlap x or g");
+    if (grad && (overlap(grad, p->ndata, x, n, es) || overlap(grad, p->ndata, g, m, es) ||
+                 overlap(grad, p->ndata, dx, n, es)))
+        return fail(HM_EINVAL, "grad must not overlap the vectors");
+    DeviceGuard gd(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // t = B' x (fwd.t), dx = H' g (leaves s = A' g in bwd.t), then the
+    // gradients of <g, H x> from x, g, t and s
+    hipError_t e = hm::launch_projection(*p, p->fwd, x, s);
+    if (e == hipSuccess) e = hm::launch_matvec(*p, p->bwd, g, dx, 1.0, 0.0, s);
+    if (e == hipSuccess && grad) e = hm::launch_adjoint_grads(*p, x, g, grad, false, s);
+    if (e != hipSuccess) return fail(HM_EHIP, std::string("hm_vjp: ") + hipGetErrorString(e));
+    return HM_OK;
+}
+
 int hm_plan_create_batch(int64_t nmat, const int64_t *nrows, const int64_t *ncols,
                          const int64_t *nblocks, const int64_t *const *blocks,
                          const double *const *data, const int64_t *ndata, int storage,

@@ -244,6 +244,24 @@ class DeviceHmat:
         _native.check(rc, "hm_adjoint")
         return (L.value if loss else None), res, dx, g
 
+    def vjp(self, x, g, grad: bool = True, stream=None):
+        """Pullback of ``y = H x`` for the cotangent ``g`` (hm_vjp).
+
+        Returns ``(dx, dg)``: ``dx = H' g`` and ``dg`` the gradient of
+        ``<g, H x>`` with respect to every arena entry in the exported
+        layout (dense C: ``g x'``, low-rank A: ``g (B'x)'``, B: ``x (A'g)'``;
+        zeros outside the leaves), or ``None`` when ``grad`` is False.
+        """
+        torch = _torch()
+        px = self._vec(x, self.ncols, "x")
+        pg = self._vec(g, self.nrows, "g")
+        dx = torch.empty(self.ncols, dtype=self.vec_dtype, device=self.device)
+        dg = torch.zeros(self.ndata, dtype=self.vec_dtype, device=self.device) if grad else None
+        rc = _native.lib().hm_vjp(self._h, px, pg, ctypes.c_void_p(dx.data_ptr()),
+                                  ctypes.c_void_p(dg.data_ptr()) if grad else None, _stream_handle(stream))
+        _native.check(rc, "hm_vjp")
+        return dx, dg
+
     # ------------------------------------------------------------------
     # numpy in, numpy out (the reference's H*y / H*X / v*H)
     def mul(self, v: np.ndarray) -> np.ndarray:
@@ -317,10 +335,19 @@ class TransposedHmat:
     __matmul__ = __mul__
 
 
-def hmatvec(dev: DeviceHmat, x):
-    """Differentiable ``H x`` for torch autograd (gradient with respect to x:
-    ``H' g``), H fixed."""
-    return _HMatVec.apply(x, dev)
+def hmatvec(dev: DeviceHmat, x, data=None):
+    """Differentiable ``H x`` for torch autograd.
+
+    The gradient with respect to x is ``H' g``.  ``data`` (optional) is a
+    tensor standing for the plan's factor arena in the exported layout (the
+    values the plan was built from, e.g. ``torch.from_numpy(flat.data)``
+    with ``requires_grad``): its gradient is the factor tangent of the
+    product, ``dC = g x'``, ``dA = g (B'x)'``, ``dB = x (A'g)'`` per leaf
+    (hm_vjp).  The product itself always uses the plan's own copy.
+    """
+    if data is not None and data.numel() != dev.ndata:
+        raise ValueError(f"data has {data.numel()} entries; the plan's arena has {dev.ndata}")
+    return _HMatVec.apply(x, data, dev)
 
 
 def _make_fn():
@@ -328,13 +355,23 @@ def _make_fn():
 
     class HMatVec(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, x, dev):
+        def forward(ctx, x, data, dev):
             ctx.dev = dev
-            return dev.matvec(x.contiguous())
+            ctx.data_meta = None if data is None else (data.dtype, data.device, data.shape)
+            x = x.contiguous()
+            ctx.save_for_backward(x)
+            return dev.matvec(x)
 
         @staticmethod
         def backward(ctx, g):
-            return ctx.dev.matvec(g.contiguous(), transpose=True), None
+            (x,) = ctx.saved_tensors
+            g = g.contiguous().to(ctx.dev.vec_dtype)
+            want = ctx.data_meta is not None and ctx.needs_input_grad[1]
+            if not want:
+                return ctx.dev.matvec(g, transpose=True), None, None
+            dx, dg = ctx.dev.vjp(x, g, grad=True)
+            dtype, device, shape = ctx.data_meta
+            return dx, dg.to(device=device, dtype=dtype).reshape(shape), None
 
     return HMatVec
 

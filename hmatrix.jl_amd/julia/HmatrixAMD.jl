@@ -5,8 +5,10 @@
 # `y' * G` run on the MI355X.  Vectors may be host arrays (copied in and out,
 # the drop-in for `H*y`, src/harithm.jl:317-329) or AMDGPU.jl `ROCArray`s
 # (device-resident, no copies).  A ChainRulesCore `rrule` differentiates
-# `G * x` with respect to x (pullback = G' * ȳ), and `adjoint_gradients`
-# returns dL/dx and the leaf/factor gradients of L = 0.5*||G x - b||^2.
+# `G * x` with respect to x (G' * ȳ) and to the factors (a Tangent whose
+# `data` is the gradient over `flatten`'s arena: C -> ȳ x', A -> ȳ (B'x)',
+# B -> x (A'ȳ)', from `hm_vjp`), and `adjoint_gradients` returns dL/dx and
+# the leaf/factor gradients of L = 0.5*||G x - b||^2.
 #
 # This file is shipped as source: Julia is not part of the build image, so
 # the tested counterpart of every call below is the Python ctypes harness
@@ -72,6 +74,7 @@ mutable struct GpuHmat
     n::Int
     storage::Symbol
     ndata::Int
+    data::Vector{Float64}     # the exported arena (the primal of the factor tangent)
 end
 
 """
@@ -87,7 +90,7 @@ function GpuHmat(H::Hmat; storage::Symbol = :f64)
                (Int64, Int64, Int64, Ptr{Int64}, Ptr{Float64}, Int64, Cint, Ref{Ptr{Cvoid}}),
                H.m, H.n, size(blocks, 2), blocks, data, length(data), STORAGE[storage], h)
     check(rc, "hm_plan_create")
-    G = GpuHmat(h[], H.m, H.n, storage, length(data))
+    G = GpuHmat(h[], H.m, H.n, storage, length(data), data)
     finalizer(G) do g
         ccall((:hm_plan_destroy, libhmat), Cint, (Ptr{Cvoid},), g.handle)
     end
@@ -133,12 +136,21 @@ function Base.:*(v::AbstractMatrix{Float64}, G::GpuHmat)        # p x m  ->  p x
     return permutedims(Array(Y))
 end
 
-# differentiation with respect to x (G fixed)
+# differentiation with respect to x and to the factors: the pullback of ȳ is
+# x̄ = G' ȳ and the gradient of <ȳ, G x> over every arena entry (hm_vjp)
 function ChainRulesCore.rrule(::typeof(*), G::GpuHmat, x::ROCVector)
-    y = G * x
+    T = vectype(G)
+    xs = T.(x)
+    y = G * xs
     function pullback(ȳ)
-        x̄ = matvec!(similar(x), G, ROCArray(unthunk(ȳ)); transpose = true)
-        return NoTangent(), NoTangent(), x̄
+        g = ROCArray(T.(unthunk(ȳ)))
+        x̄ = similar(xs, T, G.n)
+        grad = AMDGPU.zeros(T, G.ndata)
+        rc = ccall((:hm_vjp, libhmat), Cint,
+                   (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                   G.handle, pointer(xs), pointer(g), pointer(x̄), pointer(grad), stream_ptr())
+        check(rc, "hm_vjp")
+        return NoTangent(), Tangent{GpuHmat}(; data = Array(grad)), x̄
     end
     return y, pullback
 end

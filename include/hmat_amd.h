@@ -247,6 +247,21 @@ int hm_adjoint(hm_plan_t plan, const void *x, const void *b, void *res, void *dx
                double *loss, void *stream);
 
 /*
+ * Pullback of y = H x (the product of src/harithm.jl:340-354) for a
+ * cotangent g of y: dx = H' g and, when grad is not NULL, the gradient of
+ * <g, H x> with respect to every leaf entry in the caller's arena layout
+ * (dense C -> g x', low-rank A -> g (B'x)', B -> x (A'g)'; entries of grad
+ * outside the leaves are not written).  x, g, dx and grad are device
+ * arrays of the accumulation type (double for HM_F64 plans, float
+ * otherwise); dx must not overlap x or g, grad none of the vectors
+ * (HM_EINVAL).  The plan must have its H' structures (not
+ * HM_NO_TRANSPOSE).  This is the factor tangent of a ChainRulesCore rrule
+ * for `*` (no rrule exists in the reference); hm_adjoint is the special
+ * case g = H x - b.
+ */
+int hm_vjp(hm_plan_t plan, const void *x, const void *g, void *dx, void *grad, void *stream);
+
+/*
  * Capture one product (op = HM_OP_N: Y = beta*Y + alpha*H*X, HM_OP_T: with
  * H') on fixed device buffers into a HIP graph; hm_graph_launch replays it on
  * a stream with one launch.  The graph holds only kernel nodes.  The plan and

@@ -158,3 +158,87 @@ def test_adjoint_reused_buffers_ragged(hm, orc, dtype, tol):
         assert relerr(g.cpu().numpy().astype(np.float64), gr) <= tol
         out = (res, dx, g)
     d.close()
+
+
+def _vjp_closed_form(f, x, g):
+    """Gradient of <g, H x> over the arena, leaf by leaf in numpy (FP64):
+    dense C -> g x', low-rank A -> g (B'x)', B -> x (A'g)'."""
+    out = np.zeros_like(f.data)
+    for kind, r0, c0, m, n, k, oa, ob in f.blocks:
+        gr, xc = g[r0:r0 + m], x[c0:c0 + n]
+        if kind == 0:
+            out[oa:oa + m * n] = np.outer(gr, xc).ravel(order="F")
+        elif k > 0:
+            A = f.data[oa:oa + m * k].reshape((m, k), order="F")
+            B = f.data[ob:ob + n * k].reshape((n, k), order="F")
+            out[oa:oa + m * k] = np.outer(gr, B.T @ xc).ravel(order="F")
+            out[ob:ob + n * k] = np.outer(xc, A.T @ gr).ravel(order="F")
+    return out
+
+
+def test_vjp_closed_form_and_central_difference(hm, orc):
+    """hm_vjp on C1 (FP64): dx = H'g against the oracle, the factor gradient
+    against the per-leaf closed form, and <grad, d> against the central
+    difference of <g, H(p +- h d) x> through the oracle (exact for a
+    bilinear A B': the h^2 terms cancel)."""
+    import torch
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    d = hm.DeviceHmat(f, "f64")
+    rng = np.random.default_rng(21)
+    x, g = rng.standard_normal(f.ncols), rng.standard_normal(f.nrows)
+    dx, dg = d.vjp(torch.from_numpy(x).cuda(), torch.from_numpy(g).cuda())
+    dx, dg = dx.cpu().numpy(), dg.cpu().numpy()
+    assert relerr(dx, orc.matvec_t(f, g)) <= 1e-13
+    ref = _vjp_closed_form(f, x, g)
+    assert relerr(dg, ref) <= 1e-13
+    h = 1e-3
+    for seed in range(3):
+        dirn = np.random.default_rng(100 + seed).standard_normal(f.data.size)
+        fp = hm.FlatHmat(f.nrows, f.ncols, f.blocks, f.data + h * dirn)
+        fm = hm.FlatHmat(f.nrows, f.ncols, f.blocks, f.data - h * dirn)
+        cd = float(g @ (orc.matvec(fp, x) - orc.matvec(fm, x))) / (2 * h)
+        assert abs(float(dg @ dirn) - cd) <= 1e-10 * abs(cd), seed
+    d.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 2e-5)])
+def test_torch_autograd_factors(hm, orc, dtype, tol):
+    """hmatvec(dev, x, data): backward gives x.grad = H'w and data.grad = the
+    factor tangent (closed form), for L = <w, H x>; without data the arena
+    gets no gradient and x.grad is unchanged."""
+    import torch
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    d = hm.DeviceHmat(f, dtype)
+    rng = np.random.default_rng(22)
+    xn, wn = rng.standard_normal(f.ncols), rng.standard_normal(f.nrows)
+    td = d.vec_dtype
+    x = torch.from_numpy(xn).to("cuda", td).requires_grad_(True)
+    data = torch.from_numpy(f.data.copy()).cuda().requires_grad_(True)
+    w = torch.from_numpy(wn).to("cuda", td)
+    (hm.hmatvec(d, x, data) * w).sum().backward()
+    xs = x.detach().double().cpu().numpy()
+    assert relerr(x.grad.double().cpu().numpy(), orc.matvec_t(f, wn)) <= tol
+    assert data.grad.dtype == torch.float64
+    assert relerr(data.grad.cpu().numpy(), _vjp_closed_form(f, xs, wn.astype(np.float64))) <= tol
+    x2 = torch.from_numpy(xn).to("cuda", td).requires_grad_(True)
+    (hm.hmatvec(d, x2) * w).sum().backward()
+    assert torch.equal(x2.grad, x.grad)
+    d.close()
+
+
+def test_vjp_argument_errors(hm):
+    import torch
+    from hmatrix_amd._native import HmError
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    d = hm.DeviceHmat(f, "f64", {"no_transpose": True})
+    x = torch.zeros(f.ncols, dtype=torch.float64, device="cuda")
+    with pytest.raises(HmError):
+        d.vjp(x, torch.zeros(f.nrows, dtype=torch.float64, device="cuda"))
+    d.close()
+    d = hm.DeviceHmat(f, "f64")
+    with pytest.raises(ValueError):
+        hm.hmatvec(d, x, torch.zeros(3, dtype=torch.float64, device="cuda"))
+    d.close()
