@@ -503,11 +503,34 @@ def main() -> int:
     relerr_dense64 = float(np.linalg.norm(Yp[rows] - Yd) / np.linalg.norm(Yd))
     log(f"plain f64 ({S} streams): {ms64 * 1e3:.1f} us per {nrhs} rhs; relerr vs dense {relerr_dense64:.3e}; "
         f"batched vs plain {relerr_h64:.3e}")
-    # FP64 H*X through hm_matmat: panels of 8 columns share a pass over the
-    # factors (the reference's own precision for a matrix of right-hand sides)
-    ms64m = timed(torch, lambda i: p64.matmat(X64, Y64, stream=stream), args.plain_steps, 1, stream)
-    relerr_m64 = float(np.linalg.norm(Y64.cpu().numpy() - Yp) / np.linalg.norm(Yp))
-    log(f"FP64 H*X ({nrhs} rhs, hm_matmat, one stream): {ms64m * 1e3:.1f} us; vs per-column {relerr_m64:.1e}")
+    # FP64 H*X through hm_matmat (the reference's own precision for a matrix
+    # of right-hand sides): one pass over the FP64 factors per 32 columns on
+    # the f64 MFMA, two plan replicas rotated; beside it the same plans'
+    # single-column layout (panels of 8 columns on VALU)
+    pm = [hm.DeviceHmat(f, "f64") for _ in range(2)]
+    Xm = [X64, pm[0].new_cols(N, nrhs)]
+    Xm[1].copy_(X64)
+    Ym = [pm[0].new_cols(N, nrhs) for _ in range(2)]
+    ms64m = timed(torch, lambda i: pm[i % 2].matmat(Xm[i % 2], Ym[i % 2], stream=stream), args.plain_steps * 5,
+                  2, stream)
+    for q in pm:
+        q.timing(True)
+        q.timing_read()
+    timed(torch, lambda i: pm[i % 2].matmat(Xm[i % 2], Ym[i % 2], stream=stream), args.plain_steps * 5, 0, stream)
+    k64 = {"proj_ms": 0.0, "reduce_ms": 0.0, "rows_ms": 0.0, "launches": 0}
+    for q in pm:
+        r = q.timing_read()
+        for k in k64:
+            k64[k] += r[k]
+    k64us = {k: k64[f"{k}_ms"] * 1e3 / max(k64["launches"], 1) for k in ("proj", "reduce", "rows")}
+    relerr_m64 = float(np.linalg.norm(Ym[0].cpu().numpy() - Yp) / np.linalg.norm(Yp))
+    ms64p = timed(torch, lambda i: p64.matmat(X64, Y64, stream=stream), args.plain_steps, 1, stream)
+    log(f"FP64 H*X ({nrhs} rhs, hm_matmat, f64 MFMA, one stream): {ms64m * 1e3:.1f} us (proj {k64us['proj']:.1f}, "
+        f"reduce {k64us['reduce']:.1f}, rows {k64us['rows']:.1f}); vs per-column {relerr_m64:.1e}; "
+        f"panels of 8 on the single-column layout: {ms64p * 1e3:.1f} us")
+    kb64 = kernel_bytes(f, "f64", nrhs, 8)
+    for q in pm:
+        q.close()
 
     # drop-in path with host buffers (PCIe-inclusive; never the reported value)
     t0 = time.perf_counter()
@@ -592,7 +615,12 @@ def main() -> int:
         "speedup_vs_per_column_fp64": value / plain_mv_s,
         "speedup_vs_fp64_matmat": value / (nrhs / (ms64m * 1e-3)),
         "fp64_matmat": {"us_per_product": ms64m * 1e3, "H-matvec/s": nrhs / (ms64m * 1e-3),
-                        "relerr_vs_per_column": relerr_m64},
+                        "relerr_vs_per_column": relerr_m64,
+                        "GB/s": algorithmic_bytes(f, "f64", nrhs, 8) / (ms64m * 1e-3) / 1e9,
+                        "kernels_us": k64us,
+                        "rows_GBs": kb64["rows"] / (k64us["rows"] * 1e-6) / 1e9 if k64us["rows"] > 0 else None,
+                        "proj_GBs": kb64["proj"] / (k64us["proj"] * 1e-6) / 1e9 if k64us["proj"] > 0 else None,
+                        "single_layout_panels_us": ms64p * 1e3},
         "host_buffer_product_ms": pcie_s * 1e3,
         "algorithmic_bytes_per_step": bytes_b,
     }

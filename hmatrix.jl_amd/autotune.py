@@ -141,7 +141,10 @@ def autotune_batched(flat, dtype: str = "bf16x2", shapes: Iterable[int] = SHAPES
 def autotune_native(flat, dtype: str = "f64", nrhs: int = 1, reps: int = 20) -> Tuple[dict, float]:
     """The library's own sweep (``hm_autotune``, the entry point Julia and C
     callers use): ``nrhs == 1`` sweeps ``proj_chunk`` x ``row_waves``,
-    ``nrhs > 1`` the ``strip_shape``.  Returns ``(config, ms_per_product)``."""
+    ``nrhs > 1`` the ``strip_shape`` (FP64: the f64 MFMA layout against
+    panels of 8 columns on the single-column layout, whose ``proj_chunk`` x
+    ``row_waves`` it sweeps; ``no_batched`` is set when those won).
+    Returns ``(config, ms_per_product)``."""
     import ctypes
     from . import _native
     L = _native.lib()
@@ -153,6 +156,10 @@ def autotune_native(flat, dtype: str = "f64", nrhs: int = 1, reps: int = 20) -> 
                        d.ctypes.data_as(ctypes.c_void_p), d.size, _native.STORAGE[dtype], int(nrhs), int(reps),
                        ctypes.byref(cfg), ctypes.byref(ms))
     _native.check(rc, "hm_autotune")
-    out = {"proj_chunk": cfg.proj_chunk, "row_waves": cfg.row_waves} if nrhs == 1 else \
-        {"strip_shape": cfg.strip_shape}
+    if nrhs == 1:
+        out = {"proj_chunk": cfg.proj_chunk, "row_waves": cfg.row_waves}
+    elif cfg.flags & 2:        # HM_NO_BATCHED: the single-column layout won
+        out = {"proj_chunk": cfg.proj_chunk, "row_waves": cfg.row_waves, "no_batched": True}
+    else:
+        out = {"strip_shape": cfg.strip_shape}
     return out, ms.value

@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "hm_bred.h"
 #include "hm_internal.h"
 
 namespace hm {
@@ -263,69 +264,6 @@ __global__ __launch_bounds__(256) void kb_proj(int64_t gps, const uint4 *__restr
         cd = gdesc[gi * 16 + r16];
         load_v<MODE>(vslc, cd, g, vr);
         group();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// t = sum of the per-super-tile partials of one leaf, in ascending chunk
-// order (deterministic, no atomics).  Items come largest first: the first
-// nbig (more than RED_WAVE chunks) take one workgroup each -- threads split
-// into G = 256/len groups, group q sums chunks q, q+G, ... (RED_BATCH loads
-// in flight per thread) and the groups are combined in LDS in ascending
-// order; the rest take one wavefront each, lanes over the elements, all of
-// an element's chunks loaded at once and summed in order.
-constexpr int RED_BATCH = 32;
-constexpr int RED_WAVE = 32;
-__global__ __launch_bounds__(256) void kb_reduce(const BRed *__restrict__ red, int64_t nbig, int64_t nred,
-                                                 const float *__restrict__ part,
-                                                 const float *__restrict__ zeros,
-                                                 float *__restrict__ t) {
-    __shared__ float sh[256];
-    if (static_cast<int64_t>(blockIdx.x) < nbig) {
-        const BRed r = red[blockIdx.x];
-        const int per = r.len >= 256 ? 256 : r.len;    // elements per pass
-        const int G = 256 / per;
-        for (int e0 = 0; e0 < r.len; e0 += per) {
-            const int e = e0 + static_cast<int>(threadIdx.x) % per;
-            const int q = static_cast<int>(threadIdx.x) / per;
-            float acc = 0.0f;
-            if (e < r.len && q < G) {
-                const float *src = part + r.pOff + e;
-                for (int c0 = q; c0 < r.nchunk; c0 += RED_BATCH * G) {
-                    float v[RED_BATCH];
-#pragma unroll
-                    for (int u = 0; u < RED_BATCH; ++u) {
-                        const int c = c0 + u * G;
-                        v[u] = *(c < r.nchunk ? src + static_cast<int64_t>(c) * r.len : zeros);
-                    }
-#pragma unroll
-                    for (int u = 0; u < RED_BATCH; ++u) acc += v[u];
-                }
-            }
-            sh[threadIdx.x] = acc;
-            __syncthreads();
-            if (q == 0 && e < r.len) {
-                float sum = acc;
-                for (int gq = 1; gq < G; ++gq) sum += sh[gq * per + threadIdx.x];
-                t[r.tOff + e] = sum;
-            }
-            __syncthreads();
-        }
-        return;
-    }
-    const int64_t item = nbig + (static_cast<int64_t>(blockIdx.x) - nbig) * 4 + (threadIdx.x >> 6);
-    if (item >= nred) return;
-    const BRed r = red[item];
-    const int l = threadIdx.x & 63;
-    for (int e = l; e < r.len; e += 64) {
-        const float *src = part + r.pOff + e;
-        float v[RED_WAVE];
-#pragma unroll
-        for (int c = 0; c < RED_WAVE; ++c) v[c] = *(c < r.nchunk ? src + static_cast<int64_t>(c) * r.len : zeros);
-        float acc = 0.0f;
-#pragma unroll
-        for (int c = 0; c < RED_WAVE; ++c) acc += v[c];
-        t[r.tOff + e] = acc;
     }
 }
 
@@ -682,15 +620,15 @@ static hipError_t run_b(const Plan &p, const Direction &d, int ncol, const float
     hipError_t e = hipSuccess;
     if (b.nst > 0 && b.gps > 0) {
         hipLaunchKernelGGL(kb_proj<MODE>, dim3(static_cast<unsigned>(b.nst)), dim3(256), 0, s, b.gps, b.pcol,
-                           b.vslc, zf, X, ldx, d.ncols, ncol, b.t);
+                           b.vslc, zf, X, ldx, d.ncols, ncol, static_cast<float *>(b.t));
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (ev) (void)hipEventRecord(ev[0], s);
     if (b.nst > 0 && b.gps > 0 && b.nred > 0) {
         const int64_t nwg = b.nredbig + (b.nred - b.nredbig + 3) / 4;
-        hipLaunchKernelGGL(kb_reduce, dim3(static_cast<unsigned>(nwg)), dim3(256), 0, s, b.red, b.nredbig, b.nred,
-                           static_cast<const float *>(b.t) + b.tLen, zf, b.t);
+        hipLaunchKernelGGL(kb_reduce<float>, dim3(static_cast<unsigned>(nwg)), dim3(256), 0, s, b.red, b.nredbig, b.nred,
+                           static_cast<const float *>(b.t) + b.tLen, zf, static_cast<float *>(b.t));
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -37,7 +37,7 @@ struct Batched {
     int64_t nred = 0, nredbig = 0; // items; leading items with more than 32 chunks
     void *frag = nullptr;    // A fragments of every strip K-step
     void *vslc = nullptr;    // B slices per super-tile column
-    float *t = nullptr;      // [t (tLen) | partials (partLen) | NB zeros]
+    void *t = nullptr;       // [t (tLen) | partials (partLen) | NB zeros], accumulator type
     int64_t tLen = 0, partLen = 0;
 };
 
@@ -128,5 +128,10 @@ hipError_t launch_matvec_m(const Plan &p, const Direction &d, int ncol, const vo
 hipError_t launch_batched(const Plan &p, const Direction &d, int ncol, const float *X, int64_t ldx,
                           float *Y, int64_t ldy, float alpha, float beta, hipStream_t s,
                           hipEvent_t *ev = nullptr);
+// Batched FP64 plans (hm_batched64.hip): one panel of ncol <= NB
+// right-hand sides, double vectors, on the f64 MFMA.
+hipError_t launch_batched64(const Plan &p, const Direction &d, int ncol, const double *X, int64_t ldx,
+                            double *Y, int64_t ldy, double alpha, double beta, hipStream_t s,
+                            hipEvent_t *ev = nullptr);
 
 }  // namespace hm

@@ -908,8 +908,14 @@ static hipError_t run_m(const Plan &p, const Direction &d, int ncol, const void 
         const dim3 grid(static_cast<unsigned>(d.ntiles));
         const A al = static_cast<A>(alpha), be = static_cast<A>(beta);
         const int bz = beta == 0.0 ? 1 : 0;
-        hipLaunchKernelGGL((k_rows_m<St, 4>), grid, dim3(256), 0, s, d.tiles, d.mult, d.stream, d.stream_lo, x,
-                           ldx, static_cast<const A *>(t), y, ldy, ncol, al, be, bz);
+        // the plan's row_waves wavefronts per tile, as in the single-column pass
+#define HM_ROWS_M(W)                                                                                            \
+    hipLaunchKernelGGL((k_rows_m<St, W>), grid, dim3(64 * W), 0, s, d.tiles, d.mult, d.stream, d.stream_lo, x, \
+                       ldx, static_cast<const A *>(t), y, ldy, ncol, al, be, bz)
+        if (p.row_waves == 2) HM_ROWS_M(2);
+        else if (p.row_waves == 8) HM_ROWS_M(8);
+        else HM_ROWS_M(4);
+#undef HM_ROWS_M
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

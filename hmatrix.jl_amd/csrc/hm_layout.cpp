@@ -113,7 +113,8 @@ std::vector<int32_t> balance_factors(const std::vector<OB> &obs, double *d) {
 }
 
 void Arena::push(double v) {
-    if (mode == 1) f.push_back(static_cast<float>(v));
+    if (mode == 3) d.push_back(v);
+    else if (mode == 1) f.push_back(static_cast<float>(v));
     else h.push_back(f32_to_bf16_rne(static_cast<float>(v)));
 }
 
@@ -413,10 +414,10 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
         }
     }
     Arena fr;
-    fr.mode = mode == 1 ? 1 : 2;
-    const int planes = mode == 3 ? 2 : 1;
+    fr.mode = mode == HM_F64 ? 3 : mode == HM_F32 ? 1 : 2;
+    const int planes = mode == HM_BF16X2 ? 2 : 1;
     const int64_t lane_elems = 8;                  // values per lane and octet
-    const int64_t unit = mode == 1 ? 4 : 8;        // arena elements per 16 bytes
+    const int64_t unit = mode == HM_F64 ? 2 : mode == HM_F32 ? 4 : 8;   // arena elements per 16 bytes
     const int64_t zrow = (tLen + partLen) / NB;   // zero row of t (after the partials)
     std::vector<uint32_t> recs, soff(static_cast<size_t>(nstrips));
     // Pack one K-step of A fragments: rows [roff, roff + rows) of the strip,
@@ -485,9 +486,9 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
         }
         return std::make_pair(wide, nwg);
     };
-    RowShape shape = strip_shape == HM_STRIPS_WIDE ? ROW_WIDE : ROW_NARROW;
+    RowShape shape = mode == HM_F64 ? ROW_F64 : strip_shape == HM_STRIPS_WIDE ? ROW_WIDE : ROW_NARROW;
     const auto nar = windows(shape);
-    if (strip_shape == HM_STRIPS_AUTO && 10 * nar.first > nar.second) {
+    if (mode != HM_F64 && strip_shape == HM_STRIPS_AUTO && 10 * nar.first > nar.second) {
         shape = ROW_WIDE;
         windows(ROW_WIDE);
     }
@@ -614,7 +615,7 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
     if (tLen + partLen >= (int64_t(1) << 31)) throw std::length_error("t and partials exceed 2^31 floats");
     std::vector<uint32_t> pcol(static_cast<size_t>(nst * gps * 16 * 4), 0u);
     Arena vs;
-    vs.mode = mode == 1 ? 1 : 2;
+    vs.mode = fr.mode;
     for (int i = 0; i < VZERO; ++i) vs.push(0.0);
     for (int64_t st = 0; st < nst; ++st) {
         const int64_t ng = (ncolst[st] + 15) / 16;
@@ -636,7 +637,7 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
                 const int64_t base = vs.size();
                 // element offsets (both planes) stay below 2^32
                 if (base + 2 * 32 * (s1 - s0) >= (int64_t(1) << 32)) return false;
-                for (int plane = 0; plane < (mode == 3 ? 2 : 1); ++plane)
+                for (int plane = 0; plane < planes; ++plane)
                     for (int64_t r = 32 * s0; r < 32 * s1; ++r) {
                         const int64_t i = st * PROJ_SUPER + r - o.col0;
                         const double v = (i >= 0 && i < o.n) ? Bf[i + j * o.n] : 0.0;
@@ -789,7 +790,7 @@ std::string check_direction(const DirectionHost &D, int storage, int row_waves, 
 
 std::string check_batched(const BatchedHost &H, int mode, int64_t nrows, int64_t ncols) {
     const int planes = mode == HM_BF16X2 ? 2 : 1;
-    const int64_t unit = mode == HM_F32 ? 4 : 8;      // arena elements per 16-byte unit
+    const int64_t unit = mode == HM_F64 ? 2 : mode == HM_F32 ? 4 : 8;   // arena elements per 16-byte unit
     const int64_t frN = H.fr.size(), vsN = H.vs.size();
     const int64_t T = H.tLen, P = H.partLen;
     if (T % NB || P % NB) return "t or partial length is not a multiple of NB";

@@ -77,13 +77,17 @@ void layout_direction(int storage, int proj_chunk, int row_waves, int64_t nrows,
 std::string check_direction(const DirectionHost &D, int storage, int row_waves, int64_t nrows, int64_t ncols,
                             int64_t fac_elems);
 
-// Element arena of the batched layout: float, or BF16 (one plane, or the
-// split hi / lo planes of each block).
+// Element arena of the batched layout: double, float, or BF16 (one plane,
+// or the split hi / lo planes of each block).
 struct Arena {
-    int mode = 0;                 // 1 f32, 2 bf16 planes
+    int mode = 0;                 // 1 f32, 2 bf16 planes, 3 f64
     std::vector<float> f;
     std::vector<uint16_t> h;
-    int64_t size() const { return mode == 1 ? static_cast<int64_t>(f.size()) : static_cast<int64_t>(h.size()); }
+    std::vector<double> d;
+    int64_t size() const {
+        return mode == 1 ? static_cast<int64_t>(f.size())
+                         : mode == 3 ? static_cast<int64_t>(d.size()) : static_cast<int64_t>(h.size());
+    }
     void align(int64_t a) {
         while (size() % a) push(0.0);
     }
@@ -103,9 +107,11 @@ struct BatchedHost {
     Arena fr, vs;
     int64_t tLen = 0, partLen = 0;
 };
-// Batched (multi-RHS) layout of one direction for storage `mode` (1, 2, 3).
+// Batched (multi-RHS) layout of one direction for storage `mode` (enum
+// hm_storage: 0 FP64 fragments and slices, 1 FP32, 2 BF16, 3 split BF16).
 // rk factors come from `fac` (balanced FP64 arena), dense leaves from
-// `dense` at each leaf's offA.  Returns false when the matrix does not fit
+// `dense` at each leaf's offA.  FP64 plans always take the row shape
+// ROW_F64 (strip_shape is ignored).  Returns false when the matrix does not fit
 // the layout (the product then runs one right-hand side at a time).
 bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, const std::vector<OB> &obs,
                     const double *fac, const double *dense, BatchedHost &H);

@@ -115,12 +115,15 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
     B.nred = static_cast<int64_t>(H.red.size());
     B.nredbig = 0;
     while (B.nredbig < B.nred && H.red[static_cast<size_t>(B.nredbig)].nchunk > 32) ++B.nredbig;
-    B.frag = H.fr.mode == 1 ? upload_vec(p, H.fr.f) : upload_vec(p, H.fr.h);
-    B.vslc = H.vs.mode == 1 ? upload_vec(p, H.vs.f) : upload_vec(p, H.vs.h);
+    auto arena = [&](const Arena &a) {
+        return a.mode == 3 ? upload_vec(p, a.d) : a.mode == 1 ? upload_vec(p, a.f) : upload_vec(p, a.h);
+    };
+    B.frag = arena(H.fr);
+    B.vslc = arena(H.vs);
     B.tLen = H.tLen;
     B.partLen = H.partLen;
-    const size_t tsz = static_cast<size_t>(H.tLen + H.partLen + NB) * sizeof(float);
-    B.t = static_cast<float *>(dev_alloc(p, tsz));
+    const size_t tsz = static_cast<size_t>(H.tLen + H.partLen + NB) * acc_size(p.storage);
+    B.t = dev_alloc(p, tsz);
     hip_check(hipMemset(B.t, 0, tsz), "hipMemset");
     B.ready = true;
 }
@@ -150,7 +153,6 @@ void build_direction(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols
     d.tLen = D.tLen;
     d.partLen = D.partLen;
     d.t = dev_alloc(p, static_cast<size_t>(D.tLen + D.partLen) * acc_size(p.storage));
-    d.tm = dev_alloc(p, static_cast<size_t>(D.tLen + D.partLen) * MULTI_RHS * acc_size(p.storage));
     if (D.nmem > 0) {
         d.fmem.nmem = D.nmem;
         d.fmem.need = dev_upload(p, D.need);
@@ -254,13 +256,16 @@ int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void 
         (void)hipEventRecord(ev[0], s);
         return ev + 1;
     };
-    if (p->storage != HM_F64 && nrhs >= 2 && d.bat.ready && !(p->flags & HM_NO_BATCHED)) {
+    if (nrhs >= 2 && d.bat.ready && !(p->flags & HM_NO_BATCHED)) {
         for (int64_t c0 = 0; c0 < nrhs; c0 += hm::NB) {
             const int nc = static_cast<int>(std::min<int64_t>(hm::NB, nrhs - c0));
-            hipError_t e = hm::launch_batched(*p, d, nc, static_cast<const float *>(X) + c0 * ldx, ldx,
-                                              static_cast<float *>(Y) + c0 * ldy, ldy,
-                                              static_cast<float>(alpha), static_cast<float>(beta), s,
-                                              record());
+            hipError_t e =
+                p->storage == HM_F64
+                    ? hm::launch_batched64(*p, d, nc, static_cast<const double *>(X) + c0 * ldx, ldx,
+                                           static_cast<double *>(Y) + c0 * ldy, ldy, alpha, beta, s, record())
+                    : hm::launch_batched(*p, d, nc, static_cast<const float *>(X) + c0 * ldx, ldx,
+                                         static_cast<float *>(Y) + c0 * ldy, ldy, static_cast<float>(alpha),
+                                         static_cast<float>(beta), s, record());
             if (e != hipSuccess)
                 return fail(HM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
         }
@@ -477,11 +482,16 @@ int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
             }
             p->ndata = ndata;
         }
-        if (storage != HM_F64 && !(p->flags & HM_NO_BATCHED)) {
+        if (!(p->flags & HM_NO_BATCHED)) {
             build_batched(*p, p->fwd, nrows, ncols, O.fw, O.rkv.data(), data);
             if (!(p->flags & HM_NO_TRANSPOSE))
                 build_batched(*p, p->bwd, ncols, nrows, O.bw, O.rkv.data(), O.dT.data());
         }
+        // scratch of the panels of MULTI_RHS columns on the single-column
+        // layout: only directions whose several-column products run there
+        for (hm::Direction *d : {&p->fwd, &p->bwd})
+            if (!d->bat.ready && (d == &p->fwd || !(p->flags & HM_NO_TRANSPOSE)))
+                d->tm = dev_alloc(*p, static_cast<size_t>(d->tLen + d->partLen) * MULTI_RHS * acc_size(storage));
     } catch (const HipFail &h) {
         const int code = (h.e == hipErrorOutOfMemory || h.e == hipErrorMemoryAllocation) ? HM_ENOMEM : HM_EHIP;
         std::string msg = h.where + ": " + hipGetErrorString(h.e);
@@ -513,6 +523,7 @@ int hm_plan_destroy(hm_plan_t plan) {
 int64_t strip_shape_in_use(const hm::Plan &p) {
     const hm::Batched &b = p.fwd.bat;
     if (!b.ready) return HM_STRIPS_AUTO;
+    if (p.storage == HM_F64) return HM_STRIPS_F64;
     return b.shape.nw == hm::ROW_WIDE.nw ? HM_STRIPS_WIDE : HM_STRIPS_NARROW;
 }
 
@@ -881,11 +892,15 @@ int hm_autotune(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
     if (reps < 1) return fail(HM_EINVAL, "reps must be >= 1");
     if (nrhs < 1 || nrhs > hm::NB) return fail(HM_EINVAL, "nrhs must be in [1, 32]");
     if (storage < HM_F64 || storage > HM_BF16X2) return fail(HM_EINVAL, "unknown storage");
-    if (nrhs > 1 && storage == HM_F64)
-        return fail(HM_EUNSUPPORTED, "multi-right-hand-side shapes need FP32, BF16 or split-BF16 storage");
     std::vector<hm_config> cand;
     if (nrhs == 1) {
         for (int c : {256, 512, 1024, 2048, 4096})
+            for (int w : {2, 4, 8}) cand.push_back(hm_config{c, w, HM_NO_TRANSPOSE | HM_NO_BATCHED, 0});
+    } else if (storage == HM_F64) {
+        // the f64 MFMA layout (one shape), or panels of MULTI_RHS columns on
+        // the single-column layout
+        cand.push_back(hm_config{0, 0, HM_NO_TRANSPOSE, 0});
+        for (int c : {256, 1024, 4096})
             for (int w : {2, 4, 8}) cand.push_back(hm_config{c, w, HM_NO_TRANSPOSE | HM_NO_BATCHED, 0});
     } else {
         for (int sh : {HM_STRIPS_NARROW, HM_STRIPS_WIDE})
@@ -941,7 +956,9 @@ int hm_autotune(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
     if (X) (void)hipFree(X);
     if (Y) (void)hipFree(Y);
     if (rc != HM_OK) return rc;
-    bcfg.flags = 0;
+    // HM_NO_BATCHED stays when the single-column layout won a
+    // several-column product; the single-column timings never need it
+    bcfg.flags &= nrhs > 1 ? HM_NO_BATCHED : 0;
     *best = bcfg;
     if (best_ms) *best_ms = bms;
     return HM_OK;

@@ -107,11 +107,15 @@ constexpr int SROWS = 16;
 // (one workgroup of 512 rows per CU), for leaves that reach far from the
 // diagonal (strong admissibility).  xsm / 8 is even: the LDS octet stride
 // xsm / 8 + 1 is odd.
+// FP64 plans stage the window as doubles (256 bytes per row of X): bands
+// of 16 strips as in the narrow shape (8 wavefronts, two strips each), one
+// workgroup and a 132 KiB window per CU.
 struct RowShape {
     int nw, spw, xsm;
 };
 constexpr RowShape ROW_NARROW = {8, 2, 528};
 constexpr RowShape ROW_WIDE = {16, 2, 1056};
+constexpr RowShape ROW_F64 = {8, 2, 528};
 
 // Strip record, 32-bit words, 16-byte aligned:
 //   0 row0, 1 rows (<= 16), 2 nd = dense K-steps, 3 np = panel K-steps,

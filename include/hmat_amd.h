@@ -100,7 +100,10 @@ enum hm_flags {
 enum hm_strip_shape {
     HM_STRIPS_AUTO = 0,      /* narrow unless a tenth of its X windows overflow */
     HM_STRIPS_NARROW = 1,    /* 8 strips of 16 rows, X window of <= 512 rows    */
-    HM_STRIPS_WIDE = 2       /* 16 strips of 16 rows, X window of <= 1024 rows  */
+    HM_STRIPS_WIDE = 2,      /* 16 strips of 16 rows, X window of <= 1024 rows  */
+    HM_STRIPS_F64 = 3        /* reported for FP64 plans, which always take it:
+                                16 strips of 16 rows, X window of <= 512 rows
+                                of doubles, one workgroup per CU               */
 };
 
 /*
@@ -146,12 +149,16 @@ int hm_plan_create_ex(int64_t nrows, int64_t ncols, int64_t nblocks,
  * counterpart in the reference: the plan-shape autotuner, for callers that
  * tune from Julia or C).  nrhs == 1: proj_chunk in {256, 512, 1024, 2048,
  * 4096} x row_waves in {2, 4, 8} of the single-right-hand-side product;
- * 2 <= nrhs <= 32 (FP32, BF16 or split-BF16 storage): strip_shape narrow
- * and wide of the multi-right-hand-side row pass.  Each candidate plan is
- * created, run 3 times untimed and reps times timed with HIP events on a
- * stream of the call's own, then destroyed; the call allocates and frees
- * its input and output buffers (zeros).  best receives the configuration
- * (flags 0); best_ms (may be NULL) its milliseconds per product.
+ * 2 <= nrhs <= 32: FP32, BF16 or split-BF16 storage: strip_shape narrow
+ * and wide of the multi-right-hand-side row pass; FP64 storage: the f64
+ * MFMA layout, or (HM_NO_BATCHED) panels of 8 columns on the single-column
+ * layout with proj_chunk in {256, 1024, 4096} x row_waves in {2, 4, 8}.
+ * Each candidate plan is created, run 3 times untimed and reps times timed
+ * with HIP events on a stream of the call's own, then destroyed; the call
+ * allocates and frees its input and output buffers (zeros).  best receives
+ * the configuration (flags 0, or HM_NO_BATCHED when that layout won a
+ * several-column product); best_ms (may be NULL) its milliseconds per
+ * product.
  */
 int hm_autotune(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *blocks,
                 const double *data, int64_t ndata, int storage, int64_t nrhs, int reps,

@@ -90,7 +90,7 @@ void plant_direction(const DirectionHost &D, int st, int64_t nrows, int64_t ncol
 }
 
 void plant_batched(const BatchedHost &H, int st, int64_t nrows, int64_t ncols) {
-    const int64_t unit = st == HM_F32 ? 4 : 8;
+    const int64_t unit = st == HM_F64 ? 2 : st == HM_F32 ? 4 : 8;
     bool dense_done = false, panel_done = false;
     for (int64_t t = 0; t < H.nstrips && !(dense_done && panel_done); ++t) {
         const uint32_t off = H.soff[static_cast<size_t>(t)];
@@ -185,8 +185,8 @@ int main(int argc, char **argv) {
                     plant_direction(B, st, ncols, nrows, rk_elems);
                 }
             }
-        if (st == HM_F64) continue;
         for (int shape : {HM_STRIPS_AUTO, HM_STRIPS_NARROW, HM_STRIPS_WIDE}) {
+            if (st == HM_F64 && shape != HM_STRIPS_AUTO) continue;     // FP64 plans have one shape
             BatchedHost HF, HB;
             const bool okF = layout_batched(st, shape, nrows, ncols, O.fw, O.rkv.data(), data, HF);
             const bool okB = layout_batched(st, shape, ncols, nrows, O.bw, O.rkv.data(), O.dT.data(), HB);
@@ -198,7 +198,8 @@ int main(int argc, char **argv) {
             if (plant && okF) plant_batched(HF, st, nrows, ncols);
             if (plant && okB) plant_batched(HB, st, ncols, nrows);
             std::printf("%s shape %d: batched H %d (%lld strips, %s), H' %d\n", sname[st], shape, okF ? 1 : 0,
-                        static_cast<long long>(HF.nstrips), HF.shape.nw == ROW_WIDE.nw ? "wide" : "narrow",
+                        static_cast<long long>(HF.nstrips),
+                        st == HM_F64 ? "f64" : HF.shape.nw == ROW_WIDE.nw ? "wide" : "narrow",
                         okB ? 1 : 0);
         }
     }

@@ -124,7 +124,7 @@ def test_julia_binding_matches_header():
 
 def test_autotune_rejects_before_touching_the_gpu(hm):
     """hm_autotune validates its arguments first: NULL output, reps < 1,
-    nrhs outside [1, 32], and FP64 storage with several right-hand sides."""
+    nrhs outside [1, 32], an unknown storage."""
     from hmatrix_amd import _native
     L = _native.lib()
     b = np.array([[0, 0, 0, 1, 1, 0, 0, 0]], dtype=np.int64)
@@ -134,5 +134,5 @@ def test_autotune_rejects_before_touching_the_gpu(hm):
     assert L.hm_autotune(1, 1, 1, bp, dp, 1, 0, 1, 5, None, None) == -1
     assert L.hm_autotune(1, 1, 1, bp, dp, 1, 0, 1, 0, ctypes.byref(cfg), None) == -1
     assert L.hm_autotune(1, 1, 1, bp, dp, 1, 3, 33, 5, ctypes.byref(cfg), None) == -1
-    assert L.hm_autotune(1, 1, 1, bp, dp, 1, 0, 8, 5, ctypes.byref(cfg), None) == -4
-    assert "FP32" in L.hm_last_error().decode()
+    assert L.hm_autotune(1, 1, 1, bp, dp, 1, 7, 8, 5, ctypes.byref(cfg), None) == -1
+    assert "storage" in L.hm_last_error().decode()

@@ -194,6 +194,15 @@ def test_native_autotune(hm, orc):
     d.close()
     cfg2, ms2 = autotune_native(f, "bf16x2", nrhs=32, reps=3)
     assert cfg2["strip_shape"] in (1, 2) and ms2 > 0
+    # FP64 with several columns: the f64 MFMA layout or single-layout panels
+    cfg3, ms3 = autotune_native(f, "f64", nrhs=32, reps=3)
+    assert ms3 > 0 and (cfg3.get("strip_shape") == 0 or cfg3.get("no_batched")), cfg3
+    d = hm.DeviceHmat(f, "f64", cfg3)
+    X = np.random.default_rng(6).standard_normal((f.ncols, 32))
+    Xd = d.new_cols(f.ncols, 32)
+    Xd.copy_(torch.from_numpy(X))
+    assert relerr(d.matmat(Xd).cpu().numpy(), orc.matmat(f, X)) <= 1e-13
+    d.close()
 
 
 def test_graph_fp64_multi_rhs(hm):

@@ -107,6 +107,25 @@ def test_c3_fp64_matvec_vs_oracle(hm, orc, c3):
     assert err <= 1e-13
 
 
+def test_c3_fp64_batched_vs_oracle(hm, orc, c3):
+    """FP64 H*X and H'X of the 32 columns on the f64 MFMA path (one pass
+    over the FP64 factors per 32 columns) against the oracle on the sampled
+    columns (<= 1e-13)."""
+    import torch
+    par, c, H, f, R = c3
+    d = hm.DeviceHmat(f, "f64")
+    assert d.info()["batched"] == 1
+    X = d.new_cols(N3, 32)
+    X.copy_(torch.from_numpy(R).to(X.device))
+    Y = d.matmat(X).cpu().numpy()
+    Yt = d.matmat(X, transpose=True).cpu().numpy()
+    d.close()
+    e = relerr(Y[:, COLS], np.stack([orc.matvec(f, R[:, j]) for j in COLS], 1))
+    et = relerr(Yt[:, COLS], np.stack([orc.matvec_t(f, R[:, j]) for j in COLS], 1))
+    print(f"C3 FP64 batched H*X vs oracle: {e:.3e}; H'X: {et:.3e}", flush=True)
+    assert e <= 1e-13 and et <= 1e-13
+
+
 def test_c3_solve_ir(hm, c3):
     """H\\y on H + 10 I (the shift of the reference's solve test,
     test/lu.jl:25) by refinement with FP64 GPU residuals."""

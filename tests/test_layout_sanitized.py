@@ -2,8 +2,8 @@
 CPU alone with AddressSanitizer and UndefinedBehaviorSanitizer, run on the
 block tables of the GPU tests: every single-right-hand-side layout (both
 directions, four storages, projection chunks of 64 and 1024 rows, with and
-without the one-launch member order) and every batched layout (three
-storages, three row-pass shapes) is built and checked offset by offset
+without the one-launch member order) and every batched layout (four
+storages; three row-pass shapes for FP32 and BF16, FP64's own) is built and checked offset by offset
 against the arena it indexes (check_direction / check_batched).  A sanitizer
 report or a violated check fails the test.
 
