@@ -291,7 +291,9 @@ __device__ __forceinline__ void load_a(const void *frag, const float *zeros, boo
 // ---------------------------------------------------------------------------
 // Row pass.  Workgroup b (remapped so that consecutive bands run on one XCD
 // and share X rows in its L2) owns a band of NW * SPW strips; wavefront w
-// takes strips w, w + NW, ... of the band.  The band's X window
+// takes its run of the band's strips (runs balanced by K-steps at plan
+// creation: the workgroup lasts as long as its slowest wavefront).  The
+// band's X window
 // [cmin, cmin + W) (every dense piece of the band, W <= XSM rows) is staged
 // once in LDS, split into BF16 hi/lo planes in the B-operand order; a band
 // whose pieces span more rows (W = 0) reads X straight from memory.  Each
@@ -329,7 +331,8 @@ struct POp {
 
 template <int MODE, int NW, int SPW, int XSM, bool STAGED>
 __global__ __launch_bounds__(64 * NW, 4) void kb_rows(int64_t nstrips, int64_t wg_per_xcd,
-                                                   const uint32_t *__restrict__ soff,
+                                                   const uint32_t *__restrict__ soffw,
+                                                   const uint32_t *__restrict__ wpos,
                                                    const uint32_t *__restrict__ recs,
                                                    const int32_t *__restrict__ wgx,
                                                    const uint32_t *__restrict__ ulist,
@@ -358,9 +361,12 @@ __global__ __launch_bounds__(64 * NW, 4) void kb_rows(int64_t nstrips, int64_t w
     const int g = l >> 4;
     const int cmin = wgx[2 * wg], W = wgx[2 * wg + 1];
     if ((W > 0) != STAGED) return;             // the band is the other instance's
-    const int64_t first = wg * BAND + wave;    // strips first + NW j
-    const int nrun = first < nstrips ? static_cast<int>(min<int64_t>(SPW, (nstrips - first + NW - 1) / NW)) : 0;
-    const uint32_t sv = soff[first + static_cast<int64_t>(NW) * min(l, max(nrun - 1, 0))];
+    // this wavefront's run of the band's strips (slots [s0, s0 + nrun));
+    // every lane loads one slot of the band, in parallel with the run bounds
+    const uint32_t *wp = wpos + wg * (NW + 1);
+    const int s0 = static_cast<int>(wp[wave]);
+    const int nrun = static_cast<int>(wp[wave + 1]) - s0;
+    const uint32_t sv = soffw[min(wg * BAND + l % BAND, nstrips - 1)];
     constexpr int AU = MODE == MODE_F32 ? 2 : 1;     // 16-byte units per lane and plane
     constexpr uint32_t NONE = 0xffffffffu;
 
@@ -514,7 +520,7 @@ __global__ __launch_bounds__(64 * NW, 4) void kb_rows(int64_t nstrips, int64_t w
             return ch;
         }
         nat = at + 1;
-        return at + 1 < nrun ? __builtin_amdgcn_readlane(sv, (at + 1) & 63) : NONE;
+        return at + 1 < nrun ? __builtin_amdgcn_readlane(sv, (s0 + at + 1) & 63) : NONE;
     };
     auto load_rec = [&](uint32_t off, uint32_t &h0, uint32_t &h1) __attribute__((always_inline)) {
         const uint32_t *rw = recs + (off == NONE ? 0u : off);     // record 0 stands in for none
@@ -526,7 +532,7 @@ __global__ __launch_bounds__(64 * NW, 4) void kb_rows(int64_t nstrips, int64_t w
     // record's first dense A fragments, in flight with the window (its panel
     // K-step is issued after the barrier, ahead of the first dense MFMAs)
     uint32_t c0, c1, n0, n1;
-    load_rec(nrun > 0 ? __builtin_amdgcn_readlane(sv, 0) : NONE, c0, c1);
+    load_rec(nrun > 0 ? __builtin_amdgcn_readlane(sv, s0 & 63) : NONE, c0, c1);
     int nsi = 0;
     uint32_t noff = nrun > 0 ? next_of(c0, c1, 0, nsi) : NONE;
     load_rec(noff, n0, n1);
@@ -649,7 +655,7 @@ static hipError_t run_b(const Plan &p, const Direction &d, int ncol, const float
             if (!wide && !narrow) return hipErrorInvalidValue;
             const dim3 g2 = staged ? grid : dim3(static_cast<unsigned>(b.nunstaged));
 #define HM_ROWS(SH, ST)                                                                                          \
-    hipLaunchKernelGGL((kb_rows<MODE, SH.nw, SH.spw, SH.xsm, ST>), g2, block, 0, s, b.nstrips, per, b.soff, b.recs, \
+    hipLaunchKernelGGL((kb_rows<MODE, SH.nw, SH.spw, SH.xsm, ST>), g2, block, 0, s, b.nstrips, per, b.soffw, b.wpos, b.recs, \
                        b.wgx, b.ulist, b.frag, zf, tf, X, ldx, d.ncols, Y, ldy, ncol, alpha, beta, bz)
             if (wide && staged) HM_ROWS(ROW_WIDE, true);
             else if (wide) HM_ROWS(ROW_WIDE, false);

@@ -165,8 +165,8 @@ __global__ __launch_bounds__(256) void kb_proj64(int64_t gps, const uint4 *__res
 
 // ---------------------------------------------------------------------------
 // Row pass.  Workgroup b (consecutive bands on one XCD, as in the other
-// storages) owns a band of NW * SPW strips; wavefront w takes strips w,
-// w + NW, ...  The band's X window [cmin, cmin + W) is staged in LDS as
+// storages) owns a band of NW * SPW strips; wavefront w takes its run of
+// the band's strips (balanced by K-steps).  The band's X window [cmin, cmin + W) is staged in LDS as
 // doubles in the projection's pair layout (row rho, right-hand side c at
 // pair rho * 16 + c % 16, half c / 16); a band whose window does not fit
 // (W = 0) runs in the second instance and reads X from memory.  Each
@@ -176,7 +176,8 @@ __global__ __launch_bounds__(256) void kb_proj64(int64_t gps, const uint4 *__res
 // flight during the MFMAs of the current one.
 template <int NW, int SPW, int XSM, bool STAGED>
 __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg_per_xcd,
-                                                     const uint32_t *__restrict__ soff,
+                                                     const uint32_t *__restrict__ soffw,
+                                                     const uint32_t *__restrict__ wpos,
                                                      const uint32_t *__restrict__ recs,
                                                      const int32_t *__restrict__ wgx,
                                                      const uint32_t *__restrict__ ulist,
@@ -197,9 +198,12 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
     const int g = l >> 4;
     const int cmin = wgx[2 * wg], W = wgx[2 * wg + 1];
     if ((W > 0) != STAGED) return;
-    const int64_t first = wg * BAND + wave;
-    const int nrun = first < nstrips ? static_cast<int>(min<int64_t>(SPW, (nstrips - first + NW - 1) / NW)) : 0;
-    const uint32_t sv = soff[first + static_cast<int64_t>(NW) * min(l, max(nrun - 1, 0))];
+    // this wavefront's run of the band's strips (slots [s0, s0 + nrun));
+    // every lane loads one slot of the band, in parallel with the run bounds
+    const uint32_t *wp = wpos + wg * (NW + 1);
+    const int s0 = static_cast<int>(wp[wave]);
+    const int nrun = static_cast<int>(wp[wave + 1]) - s0;
+    const uint32_t sv = soffw[min(wg * BAND + l % BAND, nstrips - 1)];
     constexpr uint32_t NONE = 0xffffffffu;
 
     // window staging: unit p = column pair j + 16 * row pair q
@@ -347,7 +351,7 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
             return ch;
         }
         nat = at + 1;
-        return at + 1 < nrun ? __builtin_amdgcn_readlane(sv, (at + 1) & 63) : NONE;
+        return at + 1 < nrun ? __builtin_amdgcn_readlane(sv, (s0 + at + 1) & 63) : NONE;
     };
     auto load_rec = [&](uint32_t off, uint32_t &h0, uint32_t &h1) __attribute__((always_inline)) {
         const uint32_t *rw = recs + (off == NONE ? 0u : off);
@@ -356,7 +360,7 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
     };
 
     uint32_t c0, c1, n0, n1;
-    load_rec(nrun > 0 ? __builtin_amdgcn_readlane(sv, 0) : NONE, c0, c1);
+    load_rec(nrun > 0 ? __builtin_amdgcn_readlane(sv, s0 & 63) : NONE, c0, c1);
     int nsi = 0;
     uint32_t noff = nrun > 0 ? next_of(c0, c1, 0, nsi) : NONE;
     load_rec(noff, n0, n1);
@@ -445,13 +449,13 @@ hipError_t launch_batched64(const Plan &p, const Direction &d, int ncol, const d
         const int bz = beta == 0.0 ? 1 : 0;
         const double *fr = static_cast<const double *>(b.frag);
         hipLaunchKernelGGL((kb_rows64<SH.nw, SH.spw, SH.xsm, true>), dim3(static_cast<unsigned>(8 * per)), block, 0,
-                           s, b.nstrips, per, b.soff, b.recs, b.wgx, b.ulist, fr, zd, t, X, ldx, d.ncols, Y, ldy,
+                           s, b.nstrips, per, b.soffw, b.wpos, b.recs, b.wgx, b.ulist, fr, zd, t, X, ldx, d.ncols, Y, ldy,
                            ncol, alpha, beta, bz);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (b.nunstaged > 0) {
             hipLaunchKernelGGL((kb_rows64<SH.nw, SH.spw, SH.xsm, false>), dim3(static_cast<unsigned>(b.nunstaged)),
-                               block, 0, s, b.nstrips, per, b.soff, b.recs, b.wgx, b.ulist, fr, zd, t, X, ldx,
+                               block, 0, s, b.nstrips, per, b.soffw, b.wpos, b.recs, b.wgx, b.ulist, fr, zd, t, X, ldx,
                                d.ncols, Y, ldy, ncol, alpha, beta, bz);
             e = hipGetLastError();
             if (e != hipSuccess) return e;

@@ -25,7 +25,8 @@ struct Fused {
 struct Batched {
     bool ready = false;
     int64_t nstrips = 0;
-    uint32_t *soff = nullptr;     // word offset of each strip record
+    uint32_t *soffw = nullptr;    // word offsets of the strip records in wavefront runs
+    uint32_t *wpos = nullptr;     // run starts per band (formats: hm_types.h)
     uint32_t *recs = nullptr;     // strip records
     RowShape shape = ROW_NARROW;  // row-pass workgroup shape
     int32_t *wgx = nullptr;       // per band: X window (first row, rows; 0 rows: not staged)

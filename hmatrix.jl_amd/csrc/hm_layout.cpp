@@ -420,6 +420,7 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
     const int64_t unit = mode == HM_F64 ? 2 : mode == HM_F32 ? 4 : 8;   // arena elements per 16 bytes
     const int64_t zrow = (tLen + partLen) / NB;   // zero row of t (after the partials)
     std::vector<uint32_t> recs, soff(static_cast<size_t>(nstrips));
+    std::vector<int32_t> kcnt(static_cast<size_t>(nstrips), 0);     // K-steps per strip
     // Pack one K-step of A fragments: rows [roff, roff + rows) of the strip,
     // columns [32 s, 32 s + 32) of a block whose value is val(r, kk), klen
     // valid columns.  Returns the offset in 16-byte units.
@@ -557,6 +558,7 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
         // two record registers); a strip with more K-steps continues in a
         // chained record (word 5).
         const int32_t ntot = static_cast<int32_t>(ks.size());
+        kcnt[static_cast<size_t>(t)] = ntot;
         soff[t] = static_cast<uint32_t>(recs.size());
         int32_t k0 = 0;
         do {
@@ -658,6 +660,41 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
         return x.nchunk != y.nchunk ? x.nchunk > y.nchunk : x.len > y.len;
     });
     recs.resize(recs.size() + 128, 0u);   // the row pass reads 128 words from every record start
+    // Strips of a band to its wavefronts, longest processing time first: a
+    // workgroup lives as long as its slowest wavefront, so each strip (cost:
+    // its K-steps plus a fixed record overhead, in halves) goes to the
+    // wavefront with the least work so far; a wavefront's strips stay in
+    // row order.
+    {
+        const int64_t band = static_cast<int64_t>(shape.nw) * shape.spw;
+        const int64_t nbands = (nstrips + band - 1) / band;
+        H.soffw.assign(static_cast<size_t>(nstrips), 0u);
+        H.wpos.assign(static_cast<size_t>(nbands * (shape.nw + 1)), 0u);
+        std::vector<int64_t> idx, load(static_cast<size_t>(shape.nw));
+        std::vector<std::vector<int64_t>> mine(static_cast<size_t>(shape.nw));
+        for (int64_t w = 0; w < nbands; ++w) {
+            const int64_t t0 = w * band, t1 = std::min(nstrips, t0 + band);
+            idx.clear();
+            for (int64_t t = t0; t < t1; ++t) idx.push_back(t);
+            auto cost = [&](int64_t t) { return 2 * static_cast<int64_t>(kcnt[static_cast<size_t>(t)]) + 3; };
+            std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return cost(a) > cost(b); });
+            std::fill(load.begin(), load.end(), 0);
+            for (auto &m : mine) m.clear();
+            for (int64_t t : idx) {
+                const size_t q = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+                load[q] += cost(t);
+                mine[q].push_back(t);
+            }
+            uint32_t at = 0;
+            uint32_t *wp = H.wpos.data() + w * (shape.nw + 1);
+            for (int q = 0; q < shape.nw; ++q) {
+                std::sort(mine[static_cast<size_t>(q)].begin(), mine[static_cast<size_t>(q)].end());
+                wp[q] = at;
+                for (int64_t t : mine[static_cast<size_t>(q)]) H.soffw[static_cast<size_t>(t0 + at++)] = soff[static_cast<size_t>(t)];
+            }
+            wp[shape.nw] = at;
+        }
+    }
     H.nstrips = nstrips;
     H.shape = shape;
     H.wgx = std::move(wgx);
@@ -839,6 +876,24 @@ std::string check_batched(const BatchedHost &H, int mode, int64_t nrows, int64_t
         for (int e = 0; e < r.len; ++e)
             if (++tw[static_cast<size_t>(r.tOff + e)] > 1)
                 return violation("reduction %lld: t entry %lld written twice", q, r.tOff + e);
+    }
+    // wavefront runs: each band's slots hold the record offsets of exactly
+    // its own strips, split by wpos into ascending runs of the wavefronts
+    if (static_cast<int64_t>(H.soffw.size()) != H.nstrips ||
+        static_cast<int64_t>(H.wpos.size()) != nbands * (H.shape.nw + 1))
+        return "wavefront run tables size";
+    for (int64_t w = 0; w < nbands; ++w) {
+        const int64_t t0 = w * band, n = std::min(H.nstrips, t0 + band) - t0;
+        const uint32_t *wp = H.wpos.data() + w * (H.shape.nw + 1);
+        if (wp[0] != 0 || static_cast<int64_t>(wp[H.shape.nw]) != n)
+            return violation("band %lld: wavefront runs cover %lld of %lld strips", w, wp[H.shape.nw], n);
+        for (int q = 0; q < H.shape.nw; ++q)
+            if (wp[q + 1] < wp[q]) return violation("band %lld: wavefront %lld run ends before it starts", w, q);
+        std::vector<uint32_t> a(H.soff.begin() + t0, H.soff.begin() + t0 + n),
+            b(H.soffw.begin() + t0, H.soffw.begin() + t0 + n);
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        if (a != b) return violation("band %lld: wavefront runs are not the band's strips", w);
     }
     // row pass: strip records
     const int64_t nrec = static_cast<int64_t>(H.recs.size());

@@ -100,6 +100,7 @@ struct BatchedHost {
     int64_t nstrips = 0;
     RowShape shape = ROW_NARROW;
     std::vector<uint32_t> soff, recs;
+    std::vector<uint32_t> soffw, wpos;  // record offsets in wavefront order, run starts (hm_types.h)
     std::vector<int32_t> wgx;
     std::vector<uint32_t> pcol;        // projection column descriptors, 4 words each
     int64_t nst = 0, gps = 0;

@@ -97,7 +97,8 @@ void build_batched(hm::Plan &p, hm::Direction &d, int64_t nrows, int64_t ncols,
     if (!bad.empty()) throw std::logic_error(bad);
     Batched &B = d.bat;
     B.nstrips = H.nstrips;
-    B.soff = dev_upload(p, H.soff);
+    B.soffw = dev_upload(p, H.soffw);
+    B.wpos = dev_upload(p, H.wpos);
     B.recs = dev_upload(p, H.recs);
     B.shape = H.shape;
     B.wgx = dev_upload(p, H.wgx);

@@ -137,6 +137,11 @@ constexpr RowShape ROW_F64 = {8, 2, 528};
 //   zero-row indices when the record has no panel).  Header and
 //   descriptors fit the first 128 words; 128 zero words follow the last
 //   record: the row pass loads 128 words from each record start.
+// The row pass takes a band's strips in wavefront runs: soffw holds the
+// record offsets of band b's strips at [b * band, b * band + n) grouped by
+// wavefront, and wpos[b * (nw + 1) + w] is the first slot of wavefront w's
+// run (wpos[.. + nw] = n), so wavefront w takes slots [wpos[w], wpos[w+1]).
+// The runs balance the strips' K-steps across the wavefronts.
 constexpr int SREC_HDR = 8;
 constexpr int SREC_MAXK = (128 - SREC_HDR) / 4;
 // A fragments of one K-step: `planes` planes of nOct octets x rows x 8
