@@ -187,8 +187,14 @@ def measure_batched(hm, torch, f, RHS, storage, args, streams, rows, Yd):
     log(f"plans ({storage} x{R}) created in {time.time() - t0:.1f}s; info {info}")
     if info["batched"] != 1:
         raise RuntimeError("the batched layout was not built for this matrix")
-    Xs = [plans[0].new_cols(N, nrhs) for _ in range(R)]
-    Ys = [plans[0].new_cols(N, nrhs) for _ in range(R)]
+    pad = getattr(args, "ld_pad", 0)
+
+    def cols():
+        # column-major N x nrhs with leading dimension N + pad
+        return torch.zeros((nrhs, N + pad), dtype=torch.float32, device=dev)[:, :N].t()
+
+    Xs = [cols() for _ in range(R)]
+    Ys = [cols() for _ in range(R)]
     for Xd in Xs:
         Xd.copy_(torch.from_numpy(RHS).to(dev, torch.float32))
 

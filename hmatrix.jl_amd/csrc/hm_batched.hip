@@ -566,6 +566,8 @@ __global__ __launch_bounds__(64 * NW, 4) void kb_rows(int64_t nstrips, int64_t w
     if (nrun == 0) return;
     issue_panel(c0, c1, nd, rword(c0, c1, 3) > 0u, PN);
 
+    floatx4 p0 = {0.f, 0.f, 0.f, 0.f}, p1 = {0.f, 0.f, 0.f, 0.f};
+    int prow = -1, pR = 0;
     auto walk = [&](auto staged) __attribute__((always_inline)) {
         for (;;) {
             // dense K-steps in groups of 3 through the sets A, B, C: the A
@@ -583,28 +585,43 @@ __global__ __launch_bounds__(64 * NW, 4) void kb_rows(int64_t nstrips, int64_t w
                 __builtin_amdgcn_sched_barrier(0);
                 compute_dense(c0, c1, k + 2, k + 2 < nd, C, staged);
             }
-            // last group: its third step, then the next record's loads
+            // last group: the record after next is loaded first; the next
+            // record's first two dense K-steps go into A and B as soon as
+            // those are free, its panel into PN after this record's panel
             const int kl = 3 * (trips - 1);
-            issue_a(c0, c1, kl + 2, kl + 2 < nd, C);
-            __builtin_amdgcn_sched_barrier(0);
-            compute_dense(c0, c1, kl, kl < nd, A, staged);
-            compute_dense(c0, c1, kl + 1, kl + 1 < nd, B, staged);
-            compute_panel(PN);
             const bool more = noff != NONE;
             int nnsi = nsi;
             const uint32_t nnoff = more ? next_of(n0, n1, nsi, nnsi) : NONE;
             uint32_t m0, m1;
             load_rec(nnoff, m0, m1);
             const int nnd = more ? static_cast<int>(rword(n0, n1, 2)) : 0;
-            {
-                const int nnp = more ? static_cast<int>(rword(n0, n1, 3)) : 0;
-                issue_panel(n0, n1, nnd, more && nnp > 0, PN);
-                issue_a(n0, n1, 0, more && nnd > 0, A);
-                issue_a(n0, n1, 1, more && nnd > 1, B);
-            }
+            const int nnp = more ? static_cast<int>(rword(n0, n1, 3)) : 0;
+            issue_a(c0, c1, kl + 2, kl + 2 < nd, C);
+            __builtin_amdgcn_sched_barrier(0);
+            compute_dense(c0, c1, kl, kl < nd, A, staged);
+            issue_a(n0, n1, 0, more && nnd > 0, A);
+            __builtin_amdgcn_sched_barrier(0);
+            compute_dense(c0, c1, kl + 1, kl + 1 < nd, B, staged);
+            issue_a(n0, n1, 1, more && nnd > 1, B);
+            __builtin_amdgcn_sched_barrier(0);
+            compute_panel(PN);
+            issue_panel(n0, n1, nnd, more && nnp > 0, PN);
             __builtin_amdgcn_sched_barrier(0);
             compute_dense(c0, c1, kl + 2, kl + 2 < nd, C, staged);
-            if (rword(c0, c1, 5) == 0u) store(static_cast<int>(rword(c0, c1, 0)), static_cast<int>(rword(c0, c1, 1)));
+            if (rword(c0, c1, 5) == 0u) {
+                // the wavefront's first finished strip is written at its end,
+                // after its last loads; later strips as they finish
+                if (prow < 0) {
+                    p0 = acc0;
+                    p1 = acc1;
+                    prow = static_cast<int>(rword(c0, c1, 0));
+                    pR = static_cast<int>(rword(c0, c1, 1));
+                    acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
+                    acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+                } else {
+                    store(static_cast<int>(rword(c0, c1, 0)), static_cast<int>(rword(c0, c1, 1)));
+                }
+            }
             if (!more) break;
             c0 = n0;
             c1 = n1;
@@ -616,6 +633,11 @@ __global__ __launch_bounds__(64 * NW, 4) void kb_rows(int64_t nstrips, int64_t w
         }
     };
     walk(std::integral_constant<bool, STAGED>{});
+    if (prow >= 0) {
+        acc0 = p0;
+        acc1 = p1;
+        store(prow, pR);
+    }
 }
 
 template <int MODE>
