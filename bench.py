@@ -440,6 +440,8 @@ def main() -> int:
                     help="skip the uneven 64-matrix batch (one launch; reported in the line's \"c4\" field)")
     ap.add_argument("--no-bf16", action="store_true",
                     help="skip the plain BF16 storage line (the line's \"bf16\" field)")
+    ap.add_argument("--ld-pad", type=int, default=0,
+                    help="leading dimension of X and Y = N + this many rows (default 0)")
     ap.add_argument("--extras", action="store_true",
                     help="also time H'X, the FP64 H*y at N=65536 and the FP64 adjoint at N=65536")
     args = ap.parse_args()
@@ -484,7 +486,7 @@ def main() -> int:
         bf16_line = {"H-matvec/s": nrhs / (r16["ms"] * 1e-3), "us_per_product": r16["ms"] * 1e3,
                      "single_stream_us": r16["ms1"] * 1e3,
                      "step_frac": algorithmic_bytes(f, "bf16", nrhs, 4) / (r16["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "kb_strips_us": r16["kus"]["rows"], "kb_strips_GBs": g16, "kb_strips_frac": g16 / HBM_PEAK_GBS,
+                     "kb_rows_us": r16["kus"]["rows"], "kb_rows_GBs": g16, "kb_rows_frac": g16 / HBM_PEAK_GBS,
                      "relerr_vs_dense_fp64": r16["relerr_dense"]}
 
     # plain FP64 path: one matvec per right-hand side.  The same S-stream
@@ -596,15 +598,15 @@ def main() -> int:
                    "rk_leaves": f.counts()[1], "replicas": R, "streams": S},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(args.storage, "kb_strips"),
-                     "kernel": "kb_strips (batched row pass)", "algorithmic_bytes": kb["rows"],
+                     "traffic": profiled_traffic(args.storage, "kb_rows"),
+                     "kernel": "kb_rows (batched row pass)", "algorithmic_bytes": kb["rows"],
                      "us_per_launch": kus["rows"]},
         "kernels": {"kb_proj": {"us": kus["proj"], "algorithmic_bytes": kb["proj"], "GB/s": kgbs["proj"],
                                 "traffic": profiled_traffic(args.storage, "kb_proj")},
                     "kb_reduce": {"us": kus["reduce"], "algorithmic_bytes": 0,
                                   "traffic": profiled_traffic(args.storage, "kb_reduce")},
-                    "kb_strips": {"us": kus["rows"], "algorithmic_bytes": kb["rows"], "GB/s": kgbs["rows"],
-                                  "traffic": profiled_traffic(args.storage, "kb_strips")}},
+                    "kb_rows": {"us": kus["rows"], "algorithmic_bytes": kb["rows"], "GB/s": kgbs["rows"],
+                                "traffic": profiled_traffic(args.storage, "kb_rows")}},
         "products_in_flight": S,
         "single_stream_ms_per_step": ms1,
         "single_stream_H-matvec/s": nrhs / (ms1 * 1e-3),
@@ -626,7 +628,8 @@ def main() -> int:
                         "kernels_us": k64us,
                         "rows_GBs": kb64["rows"] / (k64us["rows"] * 1e-6) / 1e9 if k64us["rows"] > 0 else None,
                         "proj_GBs": kb64["proj"] / (k64us["proj"] * 1e-6) / 1e9 if k64us["proj"] > 0 else None,
-                        "single_layout_panels_us": ms64p * 1e3},
+                        "single_layout_panels_us": ms64p * 1e3,
+                        "traffic": {k: profiled_traffic("f64", k) for k in ("kb_proj64", "kb_reduce", "kb_rows64")}},
         "host_buffer_product_ms": pcie_s * 1e3,
         "algorithmic_bytes_per_step": bytes_b,
     }
