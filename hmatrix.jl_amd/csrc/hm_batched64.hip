@@ -13,11 +13,11 @@
 // the device).  The row pass computes Y' = X' C' per 16 right-hand sides:
 // A = X' (i = right-hand side), B = C' (j = strip row).  A K-step of 32
 // columns is eight such MFMAs: in MFMA m, lane (g, r) supplies column
-// 8 g + m of its fragment row (lane (g, r) of a fragment holds values
-// [8 g, 8 g + 8) of row r) and row src + 8 g + m of X, so the lane's eight
-// fragment values are its B operands of the eight MFMAs, and the lane ends
-// with strip row r of right-hand sides g + 4 r'.  The sum over a K-step is
-// the same as in the single-column product, in another order.
+// 4 m + g of its fragment row (FP64 fragments hold, per lane (g, r), the
+// columns 4 m + g of row r) and row src + 4 m + g of X, and the lane ends
+// with strip row r of right-hand sides g + 4 r'.  A K-step of nOct octets
+// runs MFMAs m < 2 nOct only.  The sum over a K-step is the same as in the
+// single-column product, in another order.
 //
 // Three kernels per panel, as in the other storages:
 //   kb_proj64  t_b = B_b' X[cols(b)] per 256-row super-tile (partials of
@@ -123,9 +123,20 @@ __global__ __launch_bounds__(256) void kb_proj64(int64_t gps, const uint4 *__res
     __syncthreads();
     const int ngroups = static_cast<int>(__builtin_amdgcn_readfirstlane(cd.y >> 16));
     doublex4 acc0, acc1;
+    // K-steps some column of the group reaches (the others are zero)
+    auto kmask = [&]() __attribute__((always_inline)) {
+        const int s0 = cd.y & 255, s1 = (cd.y >> 8) & 255;
+        uint32_t mk = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (__ballot(q >= s0 && q < s1) != 0) mk |= 1u << q;
+        return mk;
+    };
+    uint32_t km = kmask();
     auto half = [&](int h, const double (&v)[4][8]) __attribute__((always_inline)) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
+            if (!((km >> (4 * h + q)) & 1u)) continue;
 #pragma unroll
             for (int m = 0; m < 8; ++m) {
                 const d2u a = xs[(32 * (4 * h + q) + 8 * g + m) * 16 + r16];
@@ -147,12 +158,13 @@ __global__ __launch_bounds__(256) void kb_proj64(int64_t gps, const uint4 *__res
 #pragma unroll
         for (int r = 0; r < 4; ++r) o[r] = __shfl(static_cast<int>(cd.z), g + 4 * r, 64);
         const bool more = gi + 4 < ngroups;
-        if (more) {
+        if (more) {            // the next group's descriptor and first half (km keeps this group's mask)
             cd = gdesc[(gi + 4) * 16 + r16];
             load_v64(vslc, cd, g, 0, va);
         }
         __builtin_amdgcn_sched_barrier(0);
         half(1, vb);
+        if (more) km = kmask();
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             if (o[r] >= 0) {
@@ -170,10 +182,11 @@ __global__ __launch_bounds__(256) void kb_proj64(int64_t gps, const uint4 *__res
 // doubles in the projection's pair layout (row rho, right-hand side c at
 // pair rho * 16 + c % 16, half c / 16); a band whose window does not fit
 // (W = 0) runs in the second instance and reads X from memory.  Each
-// wavefront walks its records: a record's panel K-step (A fragments and 16
-// gathered t values per lane) is issued first, then its dense K-steps with
-// the A fragments of the next K-step (or of the next record's first) in
-// flight during the MFMAs of the current one.
+// wavefront walks its records as one stream: the A fragments of K-step
+// k + 2 are in flight during the MFMAs of K-step k, and at the end of a
+// record the next record's first two dense K-steps and its panel K-step (A
+// fragments and 16 gathered t values per lane) are issued as the operand
+// sets free up.
 template <int NW, int SPW, int XSM, bool STAGED>
 __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg_per_xcd,
                                                      const uint32_t *__restrict__ soffw,
@@ -229,14 +242,16 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
         }
     }
 
-    // this lane's 8 fragment values of K-step k of record (h0, h1)
+    // this lane's fragment values of K-step k of record (h0, h1): columns
+    // 4m + g, m < 2 nOct (the 4 units read past them belong to the next
+    // lanes and meet only skipped MFMAs)
     auto issue_a = [&](uint32_t h0, uint32_t h1, int k, bool on, double (&a)[8]) __attribute__((always_inline)) {
         const int i = SREC_HDR + 4 * k;
         const uint32_t afrag = rword64(h0, h1, i), info = rword64(h0, h1, i + 2);
         const int roff = info & 15, rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
-        const bool lon = on && g < nOct && r16 >= roff && r16 < roff + rows;
+        const bool lon = on && r16 >= roff && r16 < roff + rows;
         const d2u *p = reinterpret_cast<const d2u *>(
-            lon ? frag + 2 * (static_cast<int64_t>(afrag) + 4 * (g * rows + r16 - roff)) : zeros);
+            lon ? frag + 2 * (static_cast<int64_t>(afrag) + nOct * (g * rows + r16 - roff)) : zeros);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const d2u w = p[q];
@@ -250,9 +265,9 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
         const int i = SREC_HDR + 4 * k;
         const uint32_t afrag = rword64(h0, h1, i), src = rword64(h0, h1, i + 1), info = rword64(h0, h1, i + 2);
         const int rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
-        const bool lon = on && g < nOct && r16 < rows;
+        const bool lon = on && r16 < rows;
         const d2u *p = reinterpret_cast<const d2u *>(
-            lon ? frag + 2 * (static_cast<int64_t>(afrag) + 4 * (g * rows + r16)) : zeros);
+            lon ? frag + 2 * (static_cast<int64_t>(afrag) + nOct * (g * rows + r16)) : zeros);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const d2u w = p[q];
@@ -262,7 +277,7 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
         uint32_t ix[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int wi = (static_cast<int>(src) + 8 * g + e) & 127;
+            const int wi = (static_cast<int>(src) + 4 * e + g) & 127;
             const int ad = (wi & 63) << 2;
             const uint32_t i0 = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(ad, static_cast<int>(h0)));
             const uint32_t i1 = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(ad, static_cast<int>(h1)));
@@ -279,47 +294,35 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
     auto compute_dense = [&](uint32_t h0, uint32_t h1, int k, const double (&a)[8]) __attribute__((always_inline)) {
         const int i = SREC_HDR + 4 * k;
         const uint32_t src = rword64(h0, h1, i + 1), info = rword64(h0, h1, i + 2);
-        const int nOct = (info >> 9) & 7;
-        const bool bon = g < nOct;
+        const int nm = 2 * static_cast<int>((info >> 9) & 7);     // MFMAs of this K-step
         if constexpr (STAGED) {
-            const int rho = static_cast<int>(src) - cmin + 8 * g;
+            const int rho = static_cast<int>(src) - cmin + g;
 #pragma unroll
             for (int m = 0; m < 8; ++m) {
-                const d2u b = bon ? xs[(rho + m) * 16 + r16] : d2u{0.0, 0.0};
+                if (m >= nm) break;
+                const d2u b = xs[(rho + 4 * m) * 16 + r16];
                 acc0 = mfma64(b[0], a[m], acc0);
                 acc1 = mfma64(b[1], a[m], acc1);
             }
         } else {
-            double b0[8], b1[8];
-            const int64_t row = static_cast<int64_t>(src) + 8 * g;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                double *bb = h == 0 ? b0 : b1;
-                const int c = 16 * h + r16;
-                const double *xp = X + static_cast<int64_t>(c) * ldx + row;
-                if (bon && c < ncol && row + 8 <= ncols) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const d2u w = reinterpret_cast<const d2u *>(xp)[q];
-                        bb[2 * q] = w[0];
-                        bb[2 * q + 1] = w[1];
-                    }
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) bb[e] = (bon && c < ncol && row + e < ncols) ? xp[e] : 0.0;
-                }
-            }
+            const int64_t row = static_cast<int64_t>(src) + g;
 #pragma unroll
             for (int m = 0; m < 8; ++m) {
-                acc0 = mfma64(b0[m], a[m], acc0);
-                acc1 = mfma64(b1[m], a[m], acc1);
+                if (m >= nm) break;
+                const int64_t rm = row + 4 * m;
+                const double b0 = r16 < ncol && rm < ncols ? X[static_cast<int64_t>(r16) * ldx + rm] : 0.0;
+                const double b1 = 16 + r16 < ncol && rm < ncols ? X[static_cast<int64_t>(16 + r16) * ldx + rm] : 0.0;
+                acc0 = mfma64(b0, a[m], acc0);
+                acc1 = mfma64(b1, a[m], acc1);
             }
         }
     };
-    auto compute_panel = [&](const double (&a)[8], const double (&t0)[8], const double (&t1)[8])
-                             __attribute__((always_inline)) {
+    auto compute_panel = [&](uint32_t h0, uint32_t h1, int k, const double (&a)[8], const double (&t0)[8],
+                             const double (&t1)[8]) __attribute__((always_inline)) {
+        const int nm = 2 * static_cast<int>((rword64(h0, h1, SREC_HDR + 4 * k + 2) >> 9) & 7);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
+            if (m >= nm) break;
             acc0 = mfma64(t0[m], a[m], acc0);
             acc1 = mfma64(t1[m], a[m], acc1);
         }
@@ -365,8 +368,10 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
     uint32_t noff = nrun > 0 ? next_of(c0, c1, 0, nsi) : NONE;
     load_rec(noff, n0, n1);
     int nd = nrun > 0 ? static_cast<int>(rword64(c0, c1, 2)) : 0;
-    double A0[8], A1[8];
-    issue_a(c0, c1, 0, nd > 0, A0);
+    double A[8], B[8], C[8];
+    double PA[8], PT0[8], PT1[8];
+    issue_a(c0, c1, 0, nd > 0, A);
+    issue_a(c0, c1, 1, nd > 1, B);
     if constexpr (STAGED) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -380,38 +385,53 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
     }
     __syncthreads();
     if (nrun == 0) return;
+    issue_panel(c0, c1, nd, rword64(c0, c1, 3) > 0u, PA, PT0, PT1);
     for (;;) {
-        double PA[8], PT0[8], PT1[8];
-        const bool hasp = rword64(c0, c1, 3) > 0u;
-        issue_panel(c0, c1, nd, hasp, PA, PT0, PT1);
-        const bool more = noff != NONE;
-        const int nnd = more ? static_cast<int>(rword64(n0, n1, 2)) : 0;
-        // dense K-steps in pairs through A0 / A1; the step after the last
-        // is the next record's first
-        for (int k = 0; k < nd; k += 2) {
-            if (k + 1 < nd) issue_a(c0, c1, k + 1, true, A1);
-            else issue_a(n0, n1, 0, more && nnd > 0, A1);
+        // dense K-steps in groups of 3 through the sets A, B, C: the
+        // fragments of K-step k + 2 are loaded before the MFMAs of k (steps
+        // past nd load the zero buffer and compute nothing)
+        const int trips = (max(nd, 1) + 2) / 3;
+        for (int q = 0; q + 1 < trips; ++q) {
+            const int k = 3 * q;
+            issue_a(c0, c1, k + 2, k + 2 < nd, C);
             __builtin_amdgcn_sched_barrier(0);
-            compute_dense(c0, c1, k, A0);
-            if (k + 1 >= nd) {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) A0[e] = A1[e];
-                break;
-            }
-            if (k + 2 < nd) issue_a(c0, c1, k + 2, true, A0);
-            else issue_a(n0, n1, 0, more && nnd > 0, A0);
+            compute_dense(c0, c1, k, A);
+            issue_a(c0, c1, k + 3, k + 3 < nd, A);
             __builtin_amdgcn_sched_barrier(0);
-            compute_dense(c0, c1, k + 1, A1);
+            if (k + 1 < nd) compute_dense(c0, c1, k + 1, B);
+            issue_a(c0, c1, k + 4, k + 4 < nd, B);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 2 < nd) compute_dense(c0, c1, k + 2, C);
         }
-        if (nd == 0) issue_a(n0, n1, 0, more && nnd > 0, A0);
-        if (hasp) compute_panel(PA, PT0, PT1);
+        // last group: the record after next is loaded first; the next
+        // record's first two dense K-steps go into A and B as those free up,
+        // its panel into PA / PT after this record's panel
+        const int kl = 3 * (trips - 1);
+        const bool more = noff != NONE;
+        int nnsi = nsi;
+        const uint32_t nnoff = more ? next_of(n0, n1, nsi, nnsi) : NONE;
+        uint32_t m0, m1;
+        load_rec(nnoff, m0, m1);
+        const int nnd = more ? static_cast<int>(rword64(n0, n1, 2)) : 0;
+        const bool nnp = more && rword64(n0, n1, 3) > 0u;
+        issue_a(c0, c1, kl + 2, kl + 2 < nd, C);
+        __builtin_amdgcn_sched_barrier(0);
+        if (kl < nd) compute_dense(c0, c1, kl, A);
+        issue_a(n0, n1, 0, more && nnd > 0, A);
+        __builtin_amdgcn_sched_barrier(0);
+        if (kl + 1 < nd) compute_dense(c0, c1, kl + 1, B);
+        issue_a(n0, n1, 1, more && nnd > 1, B);
+        __builtin_amdgcn_sched_barrier(0);
+        if (rword64(c0, c1, 3) > 0u) compute_panel(c0, c1, nd, PA, PT0, PT1);
+        issue_panel(n0, n1, nnd, nnp, PA, PT0, PT1);
+        __builtin_amdgcn_sched_barrier(0);
+        if (kl + 2 < nd) compute_dense(c0, c1, kl + 2, C);
         if (rword64(c0, c1, 5) == 0u) store(static_cast<int>(rword64(c0, c1, 0)), static_cast<int>(rword64(c0, c1, 1)));
         if (!more) break;
-        int nnsi = nsi;
-        const uint32_t nnoff = next_of(n0, n1, nsi, nnsi);
         c0 = n0;
         c1 = n1;
-        load_rec(nnoff, n0, n1);
+        n0 = m0;
+        n1 = m1;
         nsi = nnsi;
         noff = nnoff;
         nd = nnd;

@@ -429,6 +429,18 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
         fr.align(unit);
         const int64_t at = fr.size() / unit;
         if (at >= (int64_t(1) << 32)) throw std::length_error("fragment arena exceeds 64 GiB");
+        if (mode == HM_F64) {
+            // FP64: lane (g, r) holds columns 4 m + g, m < 2 nOct, of row r
+            // (the f64 MFMA m takes columns 4m .. 4m+3, so a K-step of nOct
+            // octets needs MFMAs m < 2 nOct only)
+            for (int g = 0; g < 4; ++g)
+                for (int r = 0; r < rows; ++r)
+                    for (int m = 0; m < 2 * nOct; ++m) {
+                        const int64_t kk = 32 * s + 4 * m + g;
+                        fr.push(kk < klen ? val(r, kk) : 0.0);
+                    }
+            return static_cast<uint32_t>(at);
+        }
         for (int plane = 0; plane < planes; ++plane)
             for (int g = 0; g < nOct; ++g)
                 for (int r = 0; r < rows; ++r)
@@ -660,6 +672,8 @@ bool layout_batched(int mode, int strip_shape, int64_t nrows, int64_t ncols, con
         return x.nchunk != y.nchunk ? x.nchunk > y.nchunk : x.len > y.len;
     });
     recs.resize(recs.size() + 128, 0u);   // the row pass reads 128 words from every record start
+    if (mode == HM_F64)
+        for (int i = 0; i < 8; ++i) fr.push(0.0);   // FP64 lanes read 4 units from their start
     // Strips of a band to its wavefronts, longest processing time first: a
     // workgroup lives as long as its slowest wavefront, so each strip (cost:
     // its K-steps plus a fixed record overhead, in halves) goes to the
@@ -920,7 +934,9 @@ std::string check_batched(const BatchedHost &H, int mode, int64_t nrows, int64_t
                 if (kind != (panel ? SK_KIND_PANEL : 0) || nOct < 1 || nOct > 4 || rows < 1 || roff + rows > R ||
                     (panel && (roff != 0 || rows != R)))
                     return violation("strip %lld, K-step %lld: descriptor %lld", t, k, info);
-                if (static_cast<int64_t>(afrag) * unit + planes * nOct * rows * 8 > frN)
+                // FP64 lanes read 4 units from their first (nOct of their own)
+                const int64_t tail = mode == HM_F64 ? 2 * (4 - nOct) : 0;
+                if (static_cast<int64_t>(afrag) * unit + planes * nOct * rows * 8 + tail > frN)
                     return violation("strip %lld, K-step %lld: fragments at unit %lld outside", t, k, afrag);
                 if (!panel) {
                     if (src % 8 || src >= static_cast<uint64_t>(ncols))

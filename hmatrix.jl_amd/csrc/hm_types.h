@@ -146,6 +146,9 @@ constexpr int SREC_HDR = 8;
 constexpr int SREC_MAXK = (128 - SREC_HDR) / 4;
 // A fragments of one K-step: `planes` planes of nOct octets x rows x 8
 // values (lane (g, r) reads values [8g, 8g + 8) of strip row roff + r).
+// FP64 fragments: 4 x rows lanes of 2 nOct values, lane (g, r) holding
+// columns 4 m + g (m < 2 nOct) of strip row roff + r; 8 zero values follow
+// the last fragment (a lane reads 4 16-byte units from its start).
 constexpr int SK_KIND_PANEL = 1;
 
 // Projection: the rank columns (leaf b, column j) crossing a super-tile of
