@@ -1,4 +1,4 @@
-"""hmatrix_amd: MI355X H-matrix products behind Hmatrix.jl's interface.
+r"""hmatrix_amd: MI355X H-matrix products behind Hmatrix.jl's interface.
 
 Host side (FP64, numpy): ``Hparams``, ``NewHmat``, ``FullMat``, ``Hmat``,
 ``Cluster``, ``aca2``, ``info``, ``Array``/``to_fmat`` -- the names of

@@ -136,8 +136,9 @@ def test_multi_rhs_single_layout(hm, orc, dtype, tol, nrhs):
     eight columns per pass over the factors; the FP64 H*X path, and FP32 /
     split-BF16 plans built without the batched layout): H X, H' X and
     Y = beta Y + alpha H X with padded leading dimensions, against the oracle
-    column by column.  Leaves of rank up to 48 (two groups of columns per
-    projection item) and multi-chunk B factors."""
+    column by column.  Leaves of rank up to 48 (a projection item takes
+    its rank columns 4 at a time, re-reading its 8-column X slice for each
+    group) and multi-chunk B factors."""
     import torch
     from tests.helpers import wide_rank_flat
     f = wide_rank_flat(hm, nrhs)
