@@ -94,10 +94,6 @@ __device__ __forceinline__ void split8hw(const float (&v)[8], bf16x8 &hi, bf16x8
 // float boundary (the device runs in unaligned-access mode).
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
 
-__device__ __forceinline__ bf16x8 load_bf8(const uint16_t *p) {
-    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(p));
-}
-
 __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -282,8 +278,8 @@ __device__ __forceinline__ void load_a(const void *frag, const float *zeros, boo
     } else {
         const uint16_t *fb = static_cast<const uint16_t *>(frag) + 8 * (static_cast<int64_t>(afrag) + lane_unit);
         const uint16_t *zb = reinterpret_cast<const uint16_t *>(zeros);
-        ah = load_bf8(on ? fb : zb);
-        if (MODE == MODE_BF16X2) al = load_bf8(on ? fb + 8 * lo_units : zb);
+        ah = __builtin_bit_cast(bf16x8, ld_stream(on ? fb : zb));
+        if (MODE == MODE_BF16X2) al = __builtin_bit_cast(bf16x8, ld_stream(on ? fb + 8 * lo_units : zb));
         else al = __builtin_bit_cast(bf16x8, short8{0, 0, 0, 0, 0, 0, 0, 0});
     }
 }
@@ -446,13 +442,13 @@ __global__ __launch_bounds__(64 * NW, 4) void kb_rows(int64_t nstrips, int64_t w
         const bool bon = g < nOct;
         bf16x8 xh0 = zero8, xl0 = zero8, xh1 = zero8, xl1 = zero8;
         if constexpr (FROM_LDS) {
-            const int o = ((static_cast<int>(src) - cmin) >> 3) + g;
-            if (bon) {
-                xh0 = __builtin_bit_cast(bf16x8, xs[0][r16][o]);
-                xl0 = __builtin_bit_cast(bf16x8, xs[1][r16][o]);
-                xh1 = __builtin_bit_cast(bf16x8, xs[0][16 + r16][o]);
-                xl1 = __builtin_bit_cast(bf16x8, xs[1][16 + r16][o]);
-            }
+            // lanes past the piece's octets hold zero A values: they read
+            // the piece's last octet instead of branching (same window row)
+            const int o = ((static_cast<int>(src) - cmin) >> 3) + min(g, nOct - 1);
+            xh0 = __builtin_bit_cast(bf16x8, xs[0][r16][o]);
+            xl0 = __builtin_bit_cast(bf16x8, xs[1][r16][o]);
+            xh1 = __builtin_bit_cast(bf16x8, xs[0][16 + r16][o]);
+            xl1 = __builtin_bit_cast(bf16x8, xs[1][16 + r16][o]);
         } else {
             float v0[8], v1[8];
             const int64_t row = static_cast<int64_t>(src) + 8 * g;

@@ -65,10 +65,10 @@ __device__ __forceinline__ void load_v64(const double *vslc, const uint4 &cd, in
     for (int q = 0; q < 4; ++q) {
         const int s = 4 * h + q;
         const bool on = s >= s0 && s < s1;
-        const d2u *p = reinterpret_cast<const d2u *>(vslc + (on ? base + 32u * s : 0u));
+        const double *p = vslc + (on ? base + 32u * s : 0u);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const d2u w = p[e];
+            const d2u w = reinterpret_cast<const d2u *>(p)[e];
             v[q][2 * e] = w[0];
             v[q][2 * e + 1] = w[1];
         }
@@ -250,11 +250,10 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
         const uint32_t afrag = rword64(h0, h1, i), info = rword64(h0, h1, i + 2);
         const int roff = info & 15, rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
         const bool lon = on && r16 >= roff && r16 < roff + rows;
-        const d2u *p = reinterpret_cast<const d2u *>(
-            lon ? frag + 2 * (static_cast<int64_t>(afrag) + nOct * (g * rows + r16 - roff)) : zeros);
+        const double *p = lon ? frag + 2 * (static_cast<int64_t>(afrag) + nOct * (g * rows + r16 - roff)) : zeros;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const d2u w = p[q];
+            const d2u w = reinterpret_cast<const d2u *>(p)[q];
             a[2 * q] = w[0];
             a[2 * q + 1] = w[1];
         }
@@ -266,11 +265,10 @@ __global__ __launch_bounds__(64 * NW) void kb_rows64(int64_t nstrips, int64_t wg
         const uint32_t afrag = rword64(h0, h1, i), src = rword64(h0, h1, i + 1), info = rword64(h0, h1, i + 2);
         const int rows = (info >> 4) & 31, nOct = (info >> 9) & 7;
         const bool lon = on && r16 < rows;
-        const d2u *p = reinterpret_cast<const d2u *>(
-            lon ? frag + 2 * (static_cast<int64_t>(afrag) + nOct * (g * rows + r16)) : zeros);
+        const double *p = lon ? frag + 2 * (static_cast<int64_t>(afrag) + nOct * (g * rows + r16)) : zeros;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const d2u w = p[q];
+            const d2u w = reinterpret_cast<const d2u *>(p)[q];
             a[2 * q] = w[0];
             a[2 * q + 1] = w[1];
         }
