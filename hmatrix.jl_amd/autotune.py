@@ -2,7 +2,9 @@
 
 ``autotune`` sweeps ``proj_chunk`` (rows of B per projection work item) and
 ``row_waves`` (wavefronts per 64-row tile) of the single-right-hand-side
-product; ``autotune_batched`` sweeps ``strip_shape`` (workgroups of the
+product; ``autotune_batch`` the same, and the one-launch form, for a
+many-matrix plan (``DeviceHmat.batch``); ``autotune_batched`` sweeps
+``strip_shape`` (workgroups of the
 multi-right-hand-side row pass: 8 strips with a 512-row X window, or 16
 with 1024 rows).  Each times every candidate plan with HIP events on the
 current stream and returns the fastest.  Results are cached in a JSON file
@@ -93,6 +95,35 @@ def autotune(flat, dtype: str = "f64", chunks: Iterable[int] = CHUNKS, waves: It
             timings[f"chunk{c}_waves{w}"] = ms
             if ms < best_ms:
                 best, best_ms = {"proj_chunk": int(c), "row_waves": int(w)}, ms
+    _store_cache(cache, key, {"config": best, "timings": timings})
+    return best, timings
+
+
+def autotune_batch(flats, dtype: str = "f32", chunks: Iterable[int] = CHUNKS, waves: Iterable[int] = WAVES,
+                   one_launch: Iterable[bool] = (False, True), reps: int = 20,
+                   cache: Optional[str] = DEFAULT_CACHE) -> Tuple[dict, Dict[str, float]]:
+    """Return ``(best_config, timings_ms)`` for single-right-hand-side
+    products of the many-matrix plan of ``flats`` (all members per launch):
+    ``proj_chunk`` x ``row_waves`` x (three launches or ``one_launch``).  The
+    best config (with ``no_transpose`` / ``no_batched`` dropped) is what
+    ``DeviceHmat.batch(flats, dtype, config)`` takes."""
+    key = "batch:" + "|".join(workload_key(f, dtype) for f in flats)
+    hit = _load_cache(cache, key)
+    if hit:
+        return hit["config"], hit["timings"]
+    timings: Dict[str, float] = {}
+    best, best_ms = None, np.inf
+    for c in chunks:
+        for w in waves:
+            for one in one_launch:
+                cfg = {"proj_chunk": int(c), "row_waves": int(w), "no_transpose": True, "no_batched": True,
+                       "one_launch": bool(one)}
+                p = DeviceHmat.batch(flats, dtype, cfg)
+                ms = _time(p, reps)
+                p.close()
+                timings[f"chunk{c}_waves{w}_one{int(bool(one))}"] = ms
+                if ms < best_ms:
+                    best, best_ms = {"proj_chunk": int(c), "row_waves": int(w), "one_launch": bool(one)}, ms
     _store_cache(cache, key, {"config": best, "timings": timings})
     return best, timings
 

@@ -150,6 +150,26 @@ def test_autotune_picks_a_config(hm, tmp_path):
     assert best2 == best and t2 == t     # served from the cache
 
 
+def test_batch_autotune_keeps_parity(hm, orc, members, tmp_path):
+    """The many-matrix autotuner times proj_chunk x row_waves x (three
+    launches, one launch), caches the result, and the plan it picks gives
+    every member the oracle's product."""
+    import torch
+    from hmatrix_amd.autotune import autotune_batch
+    cache = str(tmp_path / "atbatch.json")
+    best, t = autotune_batch(members, "f32", chunks=(256, 1024), waves=(2, 4), reps=3, cache=cache)
+    assert len(t) == 8 and best["proj_chunk"] in (256, 1024) and best["row_waves"] in (2, 4)
+    assert autotune_batch(members, "f32", chunks=(256, 1024), waves=(2, 4), reps=3, cache=cache) == (best, t)
+    d = hm.DeviceHmat.batch(members, "f32", dict(best))
+    assert d.info()["fused"] == int(best["one_launch"])
+    rng = np.random.default_rng(17)
+    xs = [rng.standard_normal(m.ncols) for m in members]
+    y = d.matvec(torch.from_numpy(np.concatenate(xs)).float().cuda()).cpu().numpy()
+    for i, m in enumerate(members):
+        assert relerr(y[d.row_offsets[i]:d.row_offsets[i + 1]], orc.matvec(m, xs[i])) <= 2e-6, i
+    d.close()
+
+
 def test_strip_shapes_agree_and_autotune(hm, orc, tmp_path):
     """Both multi-right-hand-side row-pass shapes, forced through
     hm_config.strip_shape, match the oracle (strong admissibility, so the
