@@ -140,6 +140,59 @@ def test_no_transpose_flag(hm):
     assert e.value.code == -4
 
 
+def test_products_recorded_in_a_caller_graph(hm, orc, members):
+    """The product entry points record into a caller's stream capture
+    (torch.cuda.graph): four products of the batch (H twice, H', and
+    y = 2 H x - 0.5 y) in one graph, for both launch forms.  Every replay,
+    with eager work and synchronizations between replays, equals the eager
+    products bitwise, and the eager products equal the oracle."""
+    import torch
+    rng = np.random.default_rng(21)
+    for one in (False, True):
+        d = hm.DeviceHmat.batch(members, "f32", {"one_launch": one})
+        xh = [rng.standard_normal(d.ncols) for _ in range(2)]
+        xs = [torch.from_numpy(v).float().cuda() for v in xh]
+        z = torch.from_numpy(rng.standard_normal(d.nrows)).float().cuda()
+        y0 = torch.from_numpy(rng.standard_normal(d.nrows)).float().cuda()
+        ys = [torch.zeros(d.nrows, device="cuda") for _ in range(2)]
+        w = torch.zeros(d.ncols, device="cuda")
+        ya = torch.empty(d.nrows, device="cuda")
+        outs = (ys[0], ys[1], w, ya)
+
+        def products(stream):
+            d.matvec(xs[0], ys[0], stream=stream)
+            d.matvec(xs[1], ys[1], stream=stream)
+            d.matvec(z, w, transpose=True, stream=stream)
+            with torch.cuda.stream(stream):
+                ya.copy_(y0)
+            d.matvec(xs[0], ya, alpha=2.0, beta=-0.5, stream=stream)
+
+        products(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        ref = [t.clone() for t in outs]
+        y0h = y0.cpu().numpy().astype(np.float64)
+        for i, m in enumerate(members):
+            r0, r1 = d.row_offsets[i], d.row_offsets[i + 1]
+            assert relerr(ref[1][r0:r1].cpu().numpy(), orc.matvec(m, xh[1][d.col_offsets[i]:d.col_offsets[i + 1]])) <= 2e-6
+            want = 2.0 * orc.matvec(m, xh[0][d.col_offsets[i]:d.col_offsets[i + 1]]) - 0.5 * y0h[r0:r1]
+            assert relerr(ref[3][r0:r1].cpu().numpy(), want) <= 4e-6
+        s = torch.cuda.Stream()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            products(s)
+        for rep in range(3):
+            for t in outs:
+                t.fill_(float("nan"))          # eager work between the replays
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            for a, b in zip(outs, ref):
+                assert torch.equal(a, b), (one, rep)
+        del g
+        assert d.info()["wait_faults"] == 0
+        d.close()
+
+
 def test_autotune_picks_a_config(hm, tmp_path):
     from hmatrix_amd.autotune import autotune
     par, c, H = c1(hm, "aca2")
