@@ -14,12 +14,13 @@
 namespace hm {
 
 // 16-byte loads of data a product streams once, where one wavefront
-// instruction covers whole cache lines (BF16 A fragments: lane l reads bytes
-// [16 l, 16 l + 16) of a contiguous block; the projection's X super-tile):
+// instruction covers whole cache lines (the row pass's BF16 A fragments:
+// lane l reads bytes [16 l, 16 l + 16) of a contiguous block):
 // non-temporal, so that they do not displace from L2 the X windows that
 // neighbouring bands read again.  (Data whose lines several instructions
 // share, such as FP64 fragments read 64 bytes per lane, stays cached: a
-// non-temporal load would fetch each line once per instruction.)
+// non-temporal load would fetch each line once per instruction.  The
+// projection's X tile and B slices measured slower non-temporal.)
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4), aligned(4)));
 __device__ __forceinline__ uint4 ld_stream(const void *p) {
     const u32x4_t v = __builtin_nontemporal_load(static_cast<const u32x4_t *>(p));
