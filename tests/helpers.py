@@ -103,3 +103,49 @@ def wide_rank_flat(hm, seed: int):
                 chunks += [A.ravel(order="F"), B.ravel(order="F")]
                 off += A.size + B.size
     return hm.FlatHmat(3000, 2900, np.array(blocks, dtype=np.int64), np.concatenate(chunks))
+
+
+def random_partition_flat(hm, seed: int, nrows: int, ncols: int, side=(24, 400), p_dense: float = 0.4,
+                          kmax: int = 40):
+    """A seeded random recursive partition of an nrows x ncols matrix (not
+    an H-matrix construction: any exact tiling is a valid block table).  A
+    block larger than side[1] on a side always splits; smaller ones split
+    with probability 0.65 while both halves keep side[0] rows / columns, and
+    one split in twenty cuts anywhere (slivers of a single row or column).
+    Leaves are dense or low rank (rank 1..kmax, also above the 16 of one
+    MFMA group and the 32 of one K-step) with random factors scaled so the
+    product's entries stay O(1)."""
+    rng = np.random.default_rng(seed)
+    blocks, chunks = [], []
+    off = 0
+
+    def cut(dim):
+        if dim >= 2 and rng.random() < 0.05:
+            return int(rng.integers(1, dim))
+        return int(rng.integers(side[0], dim - side[0] + 1))
+
+    stack = [(0, 0, nrows, ncols)]
+    while stack:
+        r0, c0, m, n = stack.pop()
+        can_r, can_c = m >= 2 * side[0], n >= 2 * side[0]
+        if max(m, n) > side[1] or ((can_r or can_c) and rng.random() < 0.65):
+            if m > side[1] or (can_r and not n > side[1] and (not can_c or rng.random() < 0.5)):
+                a = cut(m)
+                stack += [(r0, c0, a, n), (r0 + a, c0, m - a, n)]
+            else:
+                a = cut(n)
+                stack += [(r0, c0, m, a), (r0, c0 + a, m, n - a)]
+            continue
+        if rng.random() < p_dense:
+            C = rng.standard_normal((m, n)) / np.sqrt(n)
+            blocks.append((0, r0, c0, m, n, 0, off, 0))
+            chunks.append(C.ravel(order="F"))
+            off += C.size
+        else:
+            k = int(rng.integers(1, kmax + 1))
+            A, B = rng.standard_normal((m, k)), rng.standard_normal((n, k)) / np.sqrt(n * k)
+            blocks.append((1, r0, c0, m, n, k, off, off + A.size))
+            chunks += [A.ravel(order="F"), B.ravel(order="F")]
+            off += A.size + B.size
+    return hm.FlatHmat(nrows, ncols, np.array(blocks, dtype=np.int64), np.concatenate(chunks))
+

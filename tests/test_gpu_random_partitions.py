@@ -1,0 +1,63 @@
+"""GPU parity on seeded random exact tilings of rectangular matrices
+(tests/helpers.py random_partition_flat): leaves of every size from single
+rows or columns to 400, dense or low rank with ranks 1-40, nrows != ncols,
+neither a multiple of the 16-row strip or the 8-column octet.  Every storage
+(FP64 <= 1e-13, FP32 and split BF16 <= 2e-5, BF16 <= 1e-2 relative) runs
+H*X and H'X against the FP64 oracle for 1, 7 and 33 right-hand sides (one
+column, a partial panel, a full panel plus one), and y = 2 H X - 0.5 Y once.
+The reference's product accepts any such tiling (src/harithm.jl:340-354
+walks whatever leaves the tree holds)."""
+import numpy as np
+import pytest
+
+from tests.helpers import random_partition_flat
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-13, "f32": 2e-5, "bf16x2": 2e-5, "bf16": 1e-2}
+
+
+def _relerr(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# (seed, leaf side bounds, dense share, largest rank): the default mix, many
+# small leaves of low rank, and large leaves that span many 256-column
+# projection super-tiles and 16-row strips
+SHAPES = [(101, (24, 400), 0.4, 40), (102, (24, 400), 0.4, 40), (103, (24, 400), 0.4, 40),
+          (104, (24, 400), 0.4, 40), (105, (8, 128), 0.7, 8), (106, (64, 1500), 0.2, 40)]
+
+
+@pytest.mark.parametrize("seed,side,p_dense,kmax", SHAPES)
+def test_random_partition_products(hm, orc, seed, side, p_dense, kmax):
+    import torch
+    rng = np.random.default_rng(seed)
+    nr, nc = int(rng.integers(300, 3000)), int(rng.integers(300, 3000))
+    f = random_partition_flat(hm, seed, nr, nc, side, p_dense, kmax)
+    print(f"seed {seed}: {nr} x {nc}, {f.nblocks} leaves", flush=True)
+    data = np.random.default_rng(seed + 1000)
+    for dtype, tol in TOL.items():
+        d = hm.DeviceHmat(f, dtype)
+        assert d.info()["batched"] == 1
+        for nrhs in (1, 7, 33):
+            X = data.standard_normal((nc, nrhs))
+            Xd = d.new_cols(nc, nrhs)
+            Xd.copy_(torch.from_numpy(X).to(Xd.dtype))
+            Y = d.matmat(Xd).cpu().numpy().astype(np.float64)
+            assert _relerr(Y, orc.matmat(f, X)) <= tol, (dtype, nrhs, "H")
+            Z = data.standard_normal((nr, nrhs))
+            Zd = d.new_cols(nr, nrhs)
+            Zd.copy_(torch.from_numpy(Z).to(Zd.dtype))
+            W = d.matmat(Zd, transpose=True).cpu().numpy().astype(np.float64)
+            ref = np.stack([orc.matvec_t(f, Z[:, i]) for i in range(nrhs)], 1)
+            assert _relerr(W, ref) <= tol, (dtype, nrhs, "H'")
+        X = data.standard_normal((nc, 32))
+        Y0 = data.standard_normal((nr, 32))
+        Xd = d.new_cols(nc, 32)
+        Xd.copy_(torch.from_numpy(X).to(Xd.dtype))
+        Yd = d.new_cols(nr, 32)
+        Yd.copy_(torch.from_numpy(Y0).to(Yd.dtype))
+        d.matmat(Xd, Yd, alpha=2.0, beta=-0.5)
+        ref = 2.0 * orc.matmat(f, X) - 0.5 * Y0
+        assert _relerr(Yd.cpu().numpy().astype(np.float64), ref) <= 2 * tol, (dtype, "alpha/beta")
+        d.close()

@@ -7,8 +7,8 @@ storages; three row-pass shapes for FP32 and BF16, FP64's own) is built and chec
 against the arena it indexes (check_direction / check_batched).  A sanitizer
 report or a violated check fails the test.
 
-The tables: the narrow-leaf comb tables and the wide-rank table of the
-batched GPU tests, C1 (src/cluster.jl:48-133 partition, weak
+The tables: the narrow-leaf comb tables, the wide-rank table and the
+random rectangular partitions of the GPU tests, C1 (src/cluster.jl:48-133 partition, weak
 admissibility), a strong-admissibility exp(-|x-y|) matrix, an uneven batch
 of five matrices concatenated block-diagonally as hm_plan_create_batch does,
 and a 5-column table, for which the batched layout must be refused (the
@@ -21,7 +21,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from tests.helpers import build, c1, c4_members, comb_flat, expk, sorted_uniform, wide_rank_flat
+from tests.helpers import build, c1, c4_members, comb_flat, expk, random_partition_flat, sorted_uniform, wide_rank_flat
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "hmatrix.jl_amd", "csrc")
@@ -91,6 +91,20 @@ def test_comb_tables(hm, checker, tmp_path, seed):
 def test_wide_rank_table(hm, checker, tmp_path):
     f = wide_rank_flat(hm, 0)
     _run(checker, tmp_path, f.nrows, f.ncols, f.blocks, f.data)
+
+
+@pytest.mark.parametrize("seed,side,p_dense,kmax", [(101, (24, 400), 0.4, 40), (102, (24, 400), 0.4, 40),
+                                                    (103, (24, 400), 0.4, 40), (104, (24, 400), 0.4, 40),
+                                                    (105, (8, 128), 0.7, 8), (106, (64, 1500), 0.2, 40)])
+def test_random_partitions(hm, checker, tmp_path, seed, side, p_dense, kmax):
+    """Seeded random exact tilings of rectangular matrices (the tables of
+    the GPU random-partition parity test): every layout checks, and every
+    planted mistake is caught."""
+    rng = np.random.default_rng(seed)
+    nr, nc = int(rng.integers(300, 3000)), int(rng.integers(300, 3000))
+    f = random_partition_flat(hm, seed, nr, nc, side, p_dense, kmax)
+    out = _run(checker, tmp_path, f.nrows, f.ncols, f.blocks, f.data)
+    assert all("batched H 1" in line for line in _batched_flags(out))
 
 
 def test_c1_weak(hm, checker, tmp_path):
