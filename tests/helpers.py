@@ -149,3 +149,19 @@ def random_partition_flat(hm, seed: int, nrows: int, ncols: int, side=(24, 400),
             off += A.size + B.size
     return hm.FlatHmat(nrows, ncols, np.array(blocks, dtype=np.int64), np.concatenate(chunks))
 
+
+def vjp_closed_form(f, x, g):
+    """Gradient of <g, H x> over the arena, leaf by leaf in numpy (FP64):
+    dense C -> g x', low-rank A -> g (B'x)', B -> x (A'g)'."""
+    out = np.zeros_like(f.data)
+    for kind, r0, c0, m, n, k, oa, ob in f.blocks:
+        gr, xc = g[r0:r0 + m], x[c0:c0 + n]
+        if kind == 0:
+            out[oa:oa + m * n] = np.outer(gr, xc).ravel(order="F")
+        elif k > 0:
+            A = f.data[oa:oa + m * k].reshape((m, k), order="F")
+            B = f.data[ob:ob + n * k].reshape((n, k), order="F")
+            out[oa:oa + m * k] = np.outer(gr, B.T @ xc).ravel(order="F")
+            out[ob:ob + n * k] = np.outer(xc, A.T @ gr).ravel(order="F")
+    return out
+

@@ -11,7 +11,7 @@ N = 65536 L is ~1e7, and subtracting the two losses directly would lose
 import numpy as np
 import pytest
 
-from tests.helpers import build, c1, cauchy, sorted_uniform, relerr
+from tests.helpers import build, c1, cauchy, sorted_uniform, relerr, vjp_closed_form
 
 pytestmark = pytest.mark.gpu
 
@@ -160,22 +160,6 @@ def test_adjoint_reused_buffers_ragged(hm, orc, dtype, tol):
     d.close()
 
 
-def _vjp_closed_form(f, x, g):
-    """Gradient of <g, H x> over the arena, leaf by leaf in numpy (FP64):
-    dense C -> g x', low-rank A -> g (B'x)', B -> x (A'g)'."""
-    out = np.zeros_like(f.data)
-    for kind, r0, c0, m, n, k, oa, ob in f.blocks:
-        gr, xc = g[r0:r0 + m], x[c0:c0 + n]
-        if kind == 0:
-            out[oa:oa + m * n] = np.outer(gr, xc).ravel(order="F")
-        elif k > 0:
-            A = f.data[oa:oa + m * k].reshape((m, k), order="F")
-            B = f.data[ob:ob + n * k].reshape((n, k), order="F")
-            out[oa:oa + m * k] = np.outer(gr, B.T @ xc).ravel(order="F")
-            out[ob:ob + n * k] = np.outer(xc, A.T @ gr).ravel(order="F")
-    return out
-
-
 def test_vjp_closed_form_and_central_difference(hm, orc):
     """hm_vjp on C1 (FP64): dx = H'g against the oracle, the factor gradient
     against the per-leaf closed form, and <grad, d> against the central
@@ -190,7 +174,7 @@ def test_vjp_closed_form_and_central_difference(hm, orc):
     dx, dg = d.vjp(torch.from_numpy(x).cuda(), torch.from_numpy(g).cuda())
     dx, dg = dx.cpu().numpy(), dg.cpu().numpy()
     assert relerr(dx, orc.matvec_t(f, g)) <= 1e-13
-    ref = _vjp_closed_form(f, x, g)
+    ref = vjp_closed_form(f, x, g)
     assert relerr(dg, ref) <= 1e-13
     h = 1e-3
     for seed in range(3):
@@ -221,7 +205,7 @@ def test_torch_autograd_factors(hm, orc, dtype, tol):
     xs = x.detach().double().cpu().numpy()
     assert relerr(x.grad.double().cpu().numpy(), orc.matvec_t(f, wn)) <= tol
     assert data.grad.dtype == torch.float64
-    assert relerr(data.grad.cpu().numpy(), _vjp_closed_form(f, xs, wn.astype(np.float64))) <= tol
+    assert relerr(data.grad.cpu().numpy(), vjp_closed_form(f, xs, wn.astype(np.float64))) <= tol
     x2 = torch.from_numpy(xn).to("cuda", td).requires_grad_(True)
     (hm.hmatvec(d, x2) * w).sum().backward()
     assert torch.equal(x2.grad, x.grad)

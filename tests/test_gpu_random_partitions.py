@@ -10,7 +10,7 @@ walks whatever leaves the tree holds)."""
 import numpy as np
 import pytest
 
-from tests.helpers import random_partition_flat
+from tests.helpers import random_partition_flat, vjp_closed_form
 
 pytestmark = pytest.mark.gpu
 
@@ -61,3 +61,44 @@ def test_random_partition_products(hm, orc, seed, side, p_dense, kmax):
         ref = 2.0 * orc.matmat(f, X) - 0.5 * Y0
         assert _relerr(Yd.cpu().numpy().astype(np.float64), ref) <= 2 * tol, (dtype, "alpha/beta")
         d.close()
+
+
+@pytest.mark.parametrize("seed,side,p_dense,kmax", SHAPES)
+def test_random_partition_vector_forms(hm, orc, seed, side, p_dense, kmax):
+    """The single-right-hand-side entry points on the same tables: hm_matvec
+    and hm_matvec_t with alpha/beta (FP64 <= 1e-13, FP32 <= 2e-6), and in
+    FP64 the adjoint (loss, residual, dx = H'(Hx - b) and the factor
+    gradients against the oracle's closed form) and hm_vjp (H'g and the
+    gradient of <g, Hx> against the per-leaf closed form), all <= 1e-13."""
+    import torch
+    rng = np.random.default_rng(seed)
+    nr, nc = int(rng.integers(300, 3000)), int(rng.integers(300, 3000))
+    f = random_partition_flat(hm, seed, nr, nc, side, p_dense, kmax)
+    data = np.random.default_rng(seed + 2000)
+    x, z = data.standard_normal(nc), data.standard_normal(nr)
+    y0, w0 = data.standard_normal(nr), data.standard_normal(nc)
+    for dtype, tol in (("f64", 1e-13), ("f32", 2e-6)):
+        d = hm.DeviceHmat(f, dtype)
+        dt = torch.float64 if dtype == "f64" else torch.float32
+        y = d.matvec(torch.from_numpy(x).to(dt).cuda()).cpu().numpy().astype(np.float64)
+        assert _relerr(y, orc.matvec(f, x)) <= tol, (dtype, "H")
+        yd = torch.from_numpy(y0).to(dt).cuda()
+        d.matvec(torch.from_numpy(x).to(dt).cuda(), yd, alpha=-1.5, beta=0.5)
+        assert _relerr(yd.cpu().numpy().astype(np.float64), 0.5 * y0 - 1.5 * orc.matvec(f, x)) <= 2 * tol
+        wd = torch.from_numpy(w0).to(dt).cuda()
+        d.matvec(torch.from_numpy(z).to(dt).cuda(), wd, alpha=2.0, beta=-1.0, transpose=True)
+        assert _relerr(wd.cpu().numpy().astype(np.float64), 2.0 * orc.matvec_t(f, z) - w0) <= 2 * tol
+        d.close()
+    d = hm.DeviceHmat(f, "f64")
+    xd, zd = torch.from_numpy(x).cuda(), torch.from_numpy(z).cuda()
+    L, res, dx, g = d.adjoint(xd, zd)
+    Lr, resr, dxr, gr = orc.adjoint(f, x, z)
+    assert abs(L - Lr) <= 1e-13 * Lr
+    assert _relerr(res.cpu().numpy(), resr) <= 1e-13
+    assert _relerr(dx.cpu().numpy(), dxr) <= 1e-13
+    assert _relerr(g.cpu().numpy(), gr) <= 1e-13
+    dxv, dg = d.vjp(xd, zd)
+    assert _relerr(dxv.cpu().numpy(), orc.matvec_t(f, z)) <= 1e-13
+    assert _relerr(dg.cpu().numpy(), vjp_closed_form(f, x, z)) <= 1e-13
+    d.close()
+
