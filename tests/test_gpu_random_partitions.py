@@ -102,3 +102,32 @@ def test_random_partition_vector_forms(hm, orc, seed, side, p_dense, kmax):
     assert _relerr(dg.cpu().numpy(), vjp_closed_form(f, x, z)) <= 1e-13
     d.close()
 
+
+@pytest.mark.parametrize("one", [False, True])
+def test_random_partition_batch(hm, orc, one):
+    """A many-matrix plan over six random tilings of different shapes (FP32,
+    all members per launch; three launches or the one-launch form with its
+    per-member counters): every member's slice of H x and of H' z against
+    the oracle (<= 2e-6), over products in a row."""
+    import torch
+    members = []
+    for i, (seed, side, p_dense, kmax) in enumerate(SHAPES):
+        rng = np.random.default_rng(seed + 500)
+        nr, nc = int(rng.integers(50, 1500)), int(rng.integers(50, 1500))
+        members.append(random_partition_flat(hm, seed + 500, nr, nc, side, p_dense, kmax))
+    d = hm.DeviceHmat.batch(members, "f32", {"one_launch": one})
+    assert d.info()["fused"] == int(one)
+    data = np.random.default_rng(77)
+    for rep in range(2):
+        xs = [data.standard_normal(m.ncols) for m in members]
+        zs = [data.standard_normal(m.nrows) for m in members]
+        y = d.matvec(torch.from_numpy(np.concatenate(xs)).float().cuda()).cpu().numpy()
+        w = d.matvec(torch.from_numpy(np.concatenate(zs)).float().cuda(), transpose=True).cpu().numpy()
+        for i, m in enumerate(members):
+            yi = y[d.row_offsets[i]:d.row_offsets[i + 1]].astype(np.float64)
+            wi = w[d.col_offsets[i]:d.col_offsets[i + 1]].astype(np.float64)
+            assert _relerr(yi, orc.matvec(m, xs[i])) <= 2e-6, (rep, i, "H")
+            assert _relerr(wi, orc.matvec_t(m, zs[i])) <= 2e-6, (rep, i, "H'")
+    assert d.info()["wait_faults"] == 0
+    d.close()
+
