@@ -193,6 +193,60 @@ def test_products_recorded_in_a_caller_graph(hm, orc, members):
         d.close()
 
 
+@pytest.mark.parametrize("dtype", ["bf16x2", "f64"])
+def test_multi_rhs_products_recorded_in_a_caller_graph(hm, orc, dtype):
+    """The multi-right-hand-side path (split BF16 and the FP64 f64-MFMA
+    layout) recorded into a caller's graph: H X1 (32 columns), H' X2 (13
+    columns) and Y = 2 H X1 - 0.5 Y0 in one graph; every replay, with eager
+    work between replays, equals the eager products bitwise, and the eager
+    products equal the oracle."""
+    import torch
+    par, c, H = c1(hm, "aca2")
+    f = hm.flatten(H)
+    d = hm.DeviceHmat(f, dtype)
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    tol = 1e-13 if dtype == "f64" else 2e-5
+    rng = np.random.default_rng(23)
+    X1h, X2h, Y0h = rng.standard_normal((f.ncols, 32)), rng.standard_normal((f.nrows, 13)), rng.standard_normal((f.nrows, 32))
+
+    def cols(a, n, k):
+        t = d.new_cols(n, k)
+        t.copy_(torch.from_numpy(a).to(dt))
+        return t
+
+    X1, X2, Y0 = cols(X1h, f.ncols, 32), cols(X2h, f.nrows, 13), cols(Y0h, f.nrows, 32)
+    Y1, W2, Ya = d.new_cols(f.nrows, 32), d.new_cols(f.ncols, 13), d.new_cols(f.nrows, 32)
+    outs = (Y1, W2, Ya)
+
+    def products(stream):
+        d.matmat(X1, Y1, stream=stream)
+        d.matmat(X2, W2, transpose=True, stream=stream)
+        with torch.cuda.stream(stream):
+            Ya.copy_(Y0)
+        d.matmat(X1, Ya, alpha=2.0, beta=-0.5, stream=stream)
+
+    products(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    ref = [t.clone() for t in outs]
+    assert relerr(ref[0].cpu().numpy(), orc.matmat(f, X1h)) <= tol
+    assert relerr(ref[1].cpu().numpy(), np.stack([orc.matvec_t(f, X2h[:, i]) for i in range(13)], 1)) <= tol
+    assert relerr(ref[2].cpu().numpy(), 2.0 * orc.matmat(f, X1h) - 0.5 * Y0h) <= 2 * tol
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        products(s)
+    for rep in range(3):
+        for t in outs:
+            t.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(outs, ref):
+            assert torch.equal(a, b), rep
+    del g
+    d.close()
+
+
 def test_autotune_picks_a_config(hm, tmp_path):
     from hmatrix_amd.autotune import autotune
     par, c, H = c1(hm, "aca2")
