@@ -33,7 +33,7 @@ def c1f(hm):
 
 
 @pytest.mark.parametrize("dtype", ["bf16x2", "f32", "bf16"])
-@pytest.mark.parametrize("nrhs", [32, 5, 40])
+@pytest.mark.parametrize("nrhs", [32, 5, 40, 97])
 def test_c1_batched(hm, orc, c1f, dtype, nrhs):
     X = np.random.default_rng(nrhs).standard_normal((c1f.ncols, nrhs))
     assert relerr(_run(hm, c1f, dtype, X), orc.matmat(c1f, X)) <= TOL[dtype]

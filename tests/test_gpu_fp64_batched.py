@@ -44,7 +44,7 @@ def c1f(hm):
     return hm.flatten(H)
 
 
-@pytest.mark.parametrize("nrhs", [2, 13, 32, 45])
+@pytest.mark.parametrize("nrhs", [2, 13, 32, 45, 70])
 def test_c1_fp64_batched(hm, orc, c1f, nrhs):
     d = _plan(hm, c1f)
     X = np.random.default_rng(nrhs).standard_normal((c1f.ncols, nrhs))

@@ -4,7 +4,8 @@ rows or columns to 400, dense or low rank with ranks 1-40, nrows != ncols,
 neither a multiple of the 16-row strip or the 8-column octet.  Every storage
 (FP64 <= 1e-13, FP32 and split BF16 <= 2e-5, BF16 <= 1e-2 relative) runs
 H*X and H'X against the FP64 oracle for 1, 7 and 33 right-hand sides (one
-column, a partial panel, a full panel plus one), and y = 2 H X - 0.5 Y once.
+column, a partial panel, a full panel plus one), y = 2 H X - 0.5 Y and
+w = -1.5 H'Z + 0.25 W once.
 The reference's product accepts any such tiling (src/harithm.jl:340-354
 walks whatever leaves the tree holds)."""
 import numpy as np
@@ -60,6 +61,15 @@ def test_random_partition_products(hm, orc, seed, side, p_dense, kmax):
         d.matmat(Xd, Yd, alpha=2.0, beta=-0.5)
         ref = 2.0 * orc.matmat(f, X) - 0.5 * Y0
         assert _relerr(Yd.cpu().numpy().astype(np.float64), ref) <= 2 * tol, (dtype, "alpha/beta")
+        Z = data.standard_normal((nr, 32))
+        W0 = data.standard_normal((nc, 32))
+        Zd = d.new_cols(nr, 32)
+        Zd.copy_(torch.from_numpy(Z).to(Zd.dtype))
+        Wd = d.new_cols(nc, 32)
+        Wd.copy_(torch.from_numpy(W0).to(Wd.dtype))
+        d.matmat(Zd, Wd, alpha=-1.5, beta=0.25, transpose=True)
+        ref = -1.5 * np.stack([orc.matvec_t(f, Z[:, i]) for i in range(32)], 1) + 0.25 * W0
+        assert _relerr(Wd.cpu().numpy().astype(np.float64), ref) <= 2 * tol, (dtype, "alpha/beta H'")
         d.close()
 
 
