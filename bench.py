@@ -231,6 +231,35 @@ def measure_batched(hm, torch, f, RHS, storage, args, streams, rows, Yd):
             "_plans": (plans, Xs, Ys)}
 
 
+def scale_report(hm, torch, args, streams) -> dict:
+    """The headline workload at four times its size (N = 2^20 points on
+    [0, 400], the same kernel and parameters; --extras): split-BF16 H*X, 32
+    right-hand sides, with the headline's schedule (plan replicas, products
+    in flight), its kb_rows rate and the relative error of 256 rows against
+    the dense FP64 kernel."""
+    import copy
+    N = 1 << 20
+    par, c, f, RHS = build_workload(hm, N, 32)
+    rows = np.sort(np.random.default_rng(5).choice(N, size=256, replace=False))
+    Yd = dense_rows_reference(par, c, RHS, rows)
+    a = copy.copy(args)
+    a.steps, a.warmup, a.ld_pad = 50, 5, 0
+    r = measure_batched(hm, torch, f, RHS, "bf16x2", a, streams, rows, Yd)
+    for p in r["_plans"][0]:
+        p.close()
+    nbytes = algorithmic_bytes(f, "bf16x2", 32, 4)
+    kb = kernel_bytes(f, "bf16x2", 32, 4)
+    out = {"scale_n": N, "scale_leaves": int(f.nblocks), "scale_us_per_product": r["ms"] * 1e3,
+           "scale_single_stream_us": r["ms1"] * 1e3, "scale_H-matvec/s": 32 / (r["ms"] * 1e-3),
+           "scale_step_frac": nbytes / (r["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "scale_kb_rows_us": r["kus"]["rows"],
+           "scale_kb_rows_frac": kb["rows"] / (r["kus"]["rows"] * 1e-6) / 1e9 / HBM_PEAK_GBS,
+           "scale_relerr_vs_dense_fp64": r["relerr_dense"]}
+    log(f"N=2^20: {out['scale_us_per_product']:.1f} us per product ({out['scale_H-matvec/s']:.0f} H-matvec/s, step "
+        f"{out['scale_step_frac']:.3f} of 8 TB/s), kb_rows {out['scale_kb_rows_frac']:.3f}")
+    return out
+
+
 def c4_report(hm, torch, stream) -> dict:
     """64 H-matrices, N log-uniform in [1000, 40000], 1/(1+((x-y)/l)^2),
     l ~ U[0.2, 5], MaxRank in {4, 8, 16}; FP32; all members per launch.
@@ -570,6 +599,10 @@ def main() -> int:
 
     c4 = None if args.no_c4 else c4_report(hm, torch, stream)
     extras = extras_report(hm, torch, stream, plans, Xs, Ys) if args.extras else None
+    if args.extras:
+        for p in plans:             # the N = 2^20 replicas take their place
+            p.close()
+        extras.update(scale_report(hm, torch, args, streams))
 
     if args.calibrate_copy_mb > 0:
         n = args.calibrate_copy_mb * (1 << 20) // 4
