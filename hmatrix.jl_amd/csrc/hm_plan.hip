@@ -367,21 +367,36 @@ int create_impl(int64_t nrows, int64_t ncols, int64_t nblocks, const int64_t *bl
             std::snprintf(msg, sizeof msg, "block %lld: unknown kind %d", (long long)b, o.kind);
             return fail(HM_EINVAL, msg);
         }
-        if (o.row0 < 0 || o.col0 < 0 || o.m < 0 || o.n < 0 || o.k < 0 || o.offA < 0 || o.offB < 0 ||
-            o.row0 + o.m > nrows || o.col0 + o.n > ncols) {
+        if (o.m < 0 || o.n < 0 || o.k < 0) {
+            std::snprintf(msg, sizeof msg, "block %lld: negative size or rank", (long long)b);
+            return fail(HM_EINVAL, msg);
+        }
+        if (o.offA < 0 || o.offB < 0) {
+            std::snprintf(msg, sizeof msg, "block %lld: negative arena offset", (long long)b);
+            return fail(HM_EINVAL, msg);
+        }
+        if (o.row0 < 0 || o.col0 < 0 || o.m > nrows || o.n > ncols || o.row0 > nrows - o.m ||
+            o.col0 > ncols - o.n) {
             std::snprintf(msg, sizeof msg, "block %lld: outside the %lldx%lld matrix", (long long)b,
                           (long long)nrows, (long long)ncols);
             return fail(HM_EINVAL, msg);
         }
+        // an a x b factor at offset off lies in the arena (no overflow for
+        // any int64 input: the product is formed only once it is <= ndata)
+        auto in_arena = [ndata](int64_t off, int64_t a, int64_t c) {
+            if (a == 0 || c == 0) return off <= ndata;
+            if (a > ndata / c) return false;
+            return off <= ndata - a * c;
+        };
         if (o.kind == KIND_DENSE) {
-            if (o.offA + o.m * o.n > ndata) {
+            if (!in_arena(o.offA, o.m, o.n)) {
                 std::snprintf(msg, sizeof msg, "block %lld: dense leaf outside the arena", (long long)b);
                 return fail(HM_EINVAL, msg);
             }
             ++ndense;
             dense_elems += o.m * o.n;
         } else {
-            if (o.k >= (int64_t(1) << 20) || o.offA + o.m * o.k > ndata || o.offB + o.n * o.k > ndata) {
+            if (o.k >= (int64_t(1) << 20) || !in_arena(o.offA, o.m, o.k) || !in_arena(o.offB, o.n, o.k)) {
                 std::snprintf(msg, sizeof msg, "block %lld: low-rank factors outside the arena",
                               (long long)b);
                 return fail(HM_EINVAL, msg);

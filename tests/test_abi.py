@@ -50,6 +50,52 @@ def test_plan_create_rejects(hm, blocks, data, shape, msg):
     assert rc == -1 and msg in err and not h.value
 
 
+@pytest.mark.parametrize("seed", [101, 105, 106])
+def test_plan_create_rejects_corrupted_random_tables(hm, seed):
+    """A seeded random exact tiling passes validation (where there is no GPU
+    the call then stops at the device with HM_EHIP; on a GPU it builds a
+    plan), and every corruption of it is rejected with HM_EINVAL and its
+    message before any device work: a leaf dropped (gap), a leaf repeated
+    (overlap), a leaf moved down one row, a dense leaf or low-rank factor
+    reaching past the arena, offsets near 2^63 (no overflow in the bounds
+    check), a negative rank, a negative offset, and the wrong matrix size."""
+    from tests.helpers import random_partition_flat
+    rng = np.random.default_rng(seed)
+    nr, nc = int(rng.integers(300, 3000)), int(rng.integers(300, 3000))
+    side, p_dense, kmax = {101: ((24, 400), 0.4, 40), 105: ((8, 128), 0.7, 8), 106: ((64, 1500), 0.2, 40)}[seed]
+    f = random_partition_flat(hm, seed, nr, nc, side, p_dense, kmax)
+    b, d = f.blocks, f.data
+    rc, err, h = _create(b, d, nr, nc)
+    assert rc in (0, -2), err
+    if rc == 0:
+        from hmatrix_amd import _native
+        _native.lib().hm_plan_destroy(h)
+    de = int(np.where(b[:, 0] == 0)[0][0])
+    lr = int(np.where(b[:, 0] == 1)[0][0])
+    big = (1 << 63) - 8
+
+    def edit(i, col, v):
+        c = b.copy()
+        c[i, col] = v
+        return c
+
+    cases = [
+        (b[1:], (nr, nc), "tile"),
+        (np.vstack([b, b[3:4]]), (nr, nc), "tile"),
+        (edit(de, 1, b[de, 1] + 1), (nr, nc), ""),                    # overlap or past the last row
+        (edit(de, 6, d.size - b[de, 3] * b[de, 4] + 1), (nr, nc), "dense leaf outside the arena"),
+        (edit(lr, 7, d.size), (nr, nc), "low-rank factors outside the arena"),
+        (edit(de, 6, big), (nr, nc), "dense leaf outside the arena"),
+        (edit(lr, 6, big), (nr, nc), "low-rank factors outside the arena"),
+        (edit(lr, 5, -1), (nr, nc), "negative size or rank"),
+        (edit(de, 6, -3), (nr, nc), "negative arena offset"),
+        (b, (nr + 1, nc), "tile"),
+    ]
+    for blocks, (m, n), msg in cases:
+        rc, err, h = _create(blocks, d, m, n)
+        assert rc == -1 and not h.value and msg in err and err, (msg, err)
+
+
 def test_plan_create_rejects_bad_storage(hm):
     rc, err, h = _create([[0, 0, 0, 1, 1, 0, 0, 0]], [1.0], 1, 1, storage=9)
     assert rc == -1 and "storage" in err
