@@ -227,6 +227,10 @@ int run_direction(hm::Plan *p, const hm::Direction &d, int64_t nrhs, const void 
     if (nrhs < 0) return fail(HM_EINVAL, "nrhs < 0");
     if (nrhs > 0 && (ldx < d.ncols || ldy < d.nrows || ldx < 1 || ldy < 1))
         return fail(HM_EINVAL, "leading dimension smaller than the vector length");
+    // the panels' element extent (nrhs - 1) * ld + n must fit a byte offset
+    constexpr int64_t MAXEL = (int64_t(1) << 62) / 8;
+    if (nrhs > 1 && (ldx > (MAXEL - d.ncols) / (nrhs - 1) || ldy > (MAXEL - d.nrows) / (nrhs - 1)))
+        return fail(HM_EINVAL, "nrhs times the leading dimension exceeds the address range");
     if (!std::isfinite(alpha) || !std::isfinite(beta))
         return fail(HM_EINVAL, "alpha and beta must be finite");
     if (nrhs > 0 && ((d.ncols > 0 && !X) || (d.nrows > 0 && !Y)))

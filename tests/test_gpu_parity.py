@@ -100,7 +100,8 @@ def test_wide_ranks_f64_and_f32(hm, orc, seed):
 def test_product_argument_errors(hm, c1_aca):
     """Entry points reject bad arguments with a negative status and a message,
     before any kernel runs: overlapping x / y, short leading dimensions,
-    negative nrhs, non-finite alpha, and adjoint outputs overlapping its
+    negative nrhs, panels whose extent overflows the address range,
+    non-finite alpha, and adjoint outputs overlapping its
     inputs or each other."""
     import ctypes
     import torch
@@ -118,6 +119,7 @@ def test_product_argument_errors(hm, c1_aca):
     err(L.hm_matvec(d._h, P(0), P(n // 2), 1.0, 0.0, None), "overlap")
     err(L.hm_matmat(d._h, 2, P(0), n - 1, P(2 * n), n, 1.0, 0.0, None), "leading dimension")
     err(L.hm_matmat(d._h, -1, P(0), n, P(2 * n), n, 1.0, 0.0, None), "nrhs")
+    err(L.hm_matmat(d._h, 1 << 40, P(0), 1 << 40, P(2 * n), n, 1.0, 0.0, None), "address range")
     err(L.hm_matvec(d._h, P(0), P(2 * n), float("nan"), 0.0, None), "finite")
     err(L.hm_adjoint(d._h, P(0), P(n), P(n // 2), P(3 * n), None, None, None), "overlap")    # res over x
     err(L.hm_adjoint(d._h, P(0), P(n), P(2 * n), P(2 * n + 7), None, None, None), "overlap")  # dx over res
