@@ -15,6 +15,7 @@
 //     layout (hm_batched.hip): dense leaves in MFMA fragment order per row
 //     tile, A rows per tile, B slices per 256-column super-tile.
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -772,11 +773,17 @@ int hm_plan_create_batch(int64_t nmat, const int64_t *nrows, const int64_t *ncol
     if (nmat <= 0 || !nrows || !ncols || !nblocks || !blocks || !data || !ndata)
         return fail(HM_EINVAL, "NULL argument or nmat <= 0");
     int64_t R = 0, C = 0, NB_ = 0, ND = 0;
+    constexpr int64_t LIM = int64_t(1) << 31;
     for (int64_t i = 0; i < nmat; ++i) {
         if (nrows[i] < 0 || ncols[i] < 0 || nblocks[i] < 0 || ndata[i] < 0)
             return fail(HM_EINVAL, "negative size in batch member");
         if ((nblocks[i] > 0 && !blocks[i]) || (ndata[i] > 0 && !data[i]))
             return fail(HM_EINVAL, "NULL array in batch member");
+        // the concatenated plan must satisfy hm_plan_create's limits (checked
+        // before any member's row table is allocated)
+        if (nrows[i] >= LIM - R || ncols[i] >= LIM - C || nblocks[i] >= LIM - NB_ ||
+            ndata[i] > std::numeric_limits<int64_t>::max() - ND)
+            return fail(HM_EINVAL, "batch too large (rows and columns must stay below 2^31)");
         R += nrows[i];
         C += ncols[i];
         NB_ += nblocks[i];
@@ -789,10 +796,19 @@ int hm_plan_create_batch(int64_t nmat, const int64_t *nrows, const int64_t *ncol
         std::vector<int64_t> cover(static_cast<size_t>(nrows[i]) + 1, 0);
         for (int64_t b = 0; b < nblocks[i]; ++b) {
             const int64_t *r = blocks[i] + 8 * b;
+            const int64_t nd = ndata[i];
+            // an a x c factor at offset off inside the member's arena, with no
+            // overflow for any int64 input
+            auto in_arena = [nd](int64_t off, int64_t a, int64_t c) {
+                if (a == 0 || c == 0) return off <= nd;
+                if (a > nd / c) return false;
+                return off <= nd - a * c;
+            };
             const bool bad = (r[0] != 0 && r[0] != 1) || r[1] < 0 || r[2] < 0 || r[3] < 0 || r[4] < 0 || r[5] < 0 ||
-                             r[6] < 0 || r[7] < 0 || r[1] + r[3] > nrows[i] || r[2] + r[4] > ncols[i] ||
-                             (r[0] == 0 ? r[6] + r[3] * r[4] > ndata[i]
-                                        : (r[6] + r[3] * r[5] > ndata[i] || r[7] + r[4] * r[5] > ndata[i]));
+                             r[6] < 0 || r[7] < 0 || r[3] > nrows[i] || r[4] > ncols[i] || r[1] > nrows[i] - r[3] ||
+                             r[2] > ncols[i] - r[4] || (r[0] == 1 && r[5] >= (int64_t(1) << 20)) ||
+                             (r[0] == 0 ? !in_arena(r[6], r[3], r[4])
+                                        : (!in_arena(r[6], r[3], r[5]) || !in_arena(r[7], r[4], r[5])));
             if (bad) {
                 char msg[120];
                 std::snprintf(msg, sizeof msg, "batch member %lld, block %lld: invalid leaf", (long long)i,

@@ -129,6 +129,31 @@ def test_batch_create_checks_each_member(hm):
         assert rc == -1 and msg in L.hm_last_error().decode() and not h.value
 
 
+def test_batch_create_rejects_oversized_and_overflowing_members(hm):
+    """hm_plan_create_batch refuses members whose sizes would take the
+    concatenated plan past 2^31 rows (before allocating anything for them)
+    and leaves whose offsets or extents overflow int64 arithmetic."""
+    from hmatrix_amd import _native
+    L = _native.lib()
+    good = np.array([[0, 0, 0, 2, 2, 0, 0, 0]], dtype=np.int64)
+    d = np.ones(4)
+    big = (1 << 63) - 8
+
+    def create(tables, nr, nc):
+        pb = (ctypes.c_void_p * 2)(*[t.ctypes.data for t in tables])
+        pd = (ctypes.c_void_p * 2)(d.ctypes.data, d.ctypes.data)
+        two = lambda a, b: (ctypes.c_int64 * 2)(a, b)
+        h = ctypes.c_void_p()
+        rc = L.hm_plan_create_batch(2, two(*nr), two(*nc), two(1, 1), pb, pd, two(4, 4), 0, None, ctypes.byref(h))
+        return rc, L.hm_last_error().decode(), h
+
+    rc, err, h = create((good, good), (2, 1 << 40), (2, 2))
+    assert rc == -1 and "too large" in err and not h.value
+    for leaf in ([0, 0, 0, 2, 2, 0, big, 0], [1, 0, 0, 2, 2, 1, 0, big], [0, big, 0, 2, 2, 0, 0, 0]):
+        rc, err, h = create((good, np.array([leaf], dtype=np.int64)), (2, 2), (2, 2))
+        assert rc == -1 and "member 1" in err and not h.value, (leaf, err)
+
+
 def test_timing_rejects_null_plan(hm):
     from hmatrix_amd import _native
     L = _native.lib()
